@@ -1,0 +1,387 @@
+"""bimp_amd -- Python host interface of the MI355X-native BIMP keypoint extractor.
+
+A thin ctypes layer over the C ABI of libbimp_hip.so (include/bimp_hip.h).  It mirrors the
+reference's host API (daniel-koudouna/cuda-bimp): ``CudaStuff`` (cuda_bimp.hpp:92-100),
+``get_keypoints`` (cuda_bimp.hpp:107-111), ``make_lambdas_log`` (cuda_bimp.hpp:146),
+``snap_transform_size`` (cuda_bimp.hpp:140) and the ``Context`` class (bimp.hpp:19-51).
+
+There is no CPU fallback: every compute call runs the HIP kernels, and importing this module
+raises if the shared library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbimp_hip.so")
+
+NUM_ORI = 8
+KP_VECTOR_LENGTH = 7
+KP_MAX_POINTS = 8000
+
+OK, W_OVERFLOW, E_ARG, E_ALLOC, E_DEVICE, E_UNSUPPORTED = 0, 1, -1, -2, -3, -4
+MAP_DOUBLE, MAP_SINGLE, MAP_COMPLEX = 0, 1, 2
+FLAG_PROFILE = 1
+
+# SURF_CUDA keypoint-matrix rows (utils.cpp:109-150)
+X_ROW, Y_ROW, LAPLACIAN_ROW, OCTAVE_ROW, SIZE_ROW, ANGLE_ROW, HESSIAN_ROW = range(7)
+
+KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                           ("response", "<f4"), ("octave", "<i4"), ("class_id", "<i4")])
+
+
+class BimpError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"bimp status {status}: {msg}")
+        self.status = status
+
+
+class Options(C.Structure):
+    _fields_ = [("gamma", C.c_float), ("sigma_ratio", C.c_float), ("support", C.c_float),
+                ("pyr_threshold", C.c_float), ("nms_threshold", C.c_double),
+                ("input_scale", C.c_float), ("flags", C.c_int)]
+
+
+class ScaleInfo(C.Structure):
+    _fields_ = [("lambda2", C.c_float), ("lam", C.c_float), ("pyrstep", C.c_float),
+                ("kernel_size", C.c_int), ("kernel_centre", C.c_int), ("border", C.c_int),
+                ("level", C.c_int), ("data_h", C.c_int), ("data_w", C.c_int),
+                ("fft_h", C.c_int), ("fft_w", C.c_int), ("map_h", C.c_int),
+                ("map_w", C.c_int), ("map_pitch", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ProfileEntry(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int), ("total_ms", C.c_double),
+                ("bytes", C.c_double)]
+
+
+# Every symbol include/bimp_hip.h declares (checked by the tests).
+EXPORTED = [
+    "bimp_default_options", "bimp_snap_transform_size", "bimp_make_lambdas_log", "bimp_create",
+    "bimp_destroy", "bimp_num_scales", "bimp_get_scale_info", "bimp_detect_u8",
+    "bimp_detect_u8_matrix", "bimp_filter_u8", "bimp_cells", "bimp_keypoints", "bimp_get_map",
+    "bimp_set_map", "bimp_keypoints_to_matrix", "bimp_matrix_to_keypoints", "bimp_last_error",
+    "bimp_status_string", "bimp_get_profile", "bimp_reset_profile", "bimp_set_flags",
+    "bimp_device_count", "bimp_device_malloc", "bimp_device_free", "bimp_memcpy_h2d",
+    "bimp_memcpy_d2h", "bimp_stream_create", "bimp_stream_destroy", "bimp_stream_sync",
+    "bimp_time_detect", "bimp_copy_kernel", "bimp_synth_frame",
+]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C cuda-bimp_amd` "
+                          "or __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, I, SZ = C.c_void_p, C.c_int, C.c_size_t
+    sig = {
+        "bimp_default_options": (None, [C.POINTER(Options)]),
+        "bimp_snap_transform_size": (I, [I]),
+        "bimp_make_lambdas_log": (I, [C.c_float, C.c_float, I, P, I]),
+        "bimp_create": (I, [I, I, P, I, C.POINTER(Options), C.POINTER(P)]),
+        "bimp_destroy": (I, [P]),
+        "bimp_num_scales": (I, [P]),
+        "bimp_get_scale_info": (I, [P, I, C.POINTER(ScaleInfo)]),
+        "bimp_detect_u8": (I, [P, P, SZ, I, P, I, I, C.POINTER(I), P]),
+        "bimp_detect_u8_matrix": (I, [P, P, SZ, I, P, I, I, I, C.POINTER(I), P]),
+        "bimp_filter_u8": (I, [P, P, SZ, I, P]),
+        "bimp_cells": (I, [P, P]),
+        "bimp_keypoints": (I, [P, P, I, I, C.POINTER(I), P]),
+        "bimp_get_map": (I, [P, I, I, I, P, SZ, I, P]),
+        "bimp_set_map": (I, [P, I, I, I, P, SZ, I, P]),
+        "bimp_keypoints_to_matrix": (I, [P, I, P, I]),
+        "bimp_matrix_to_keypoints": (I, [P, I, I, P]),
+        "bimp_last_error": (C.c_char_p, [P]),
+        "bimp_status_string": (C.c_char_p, [I]),
+        "bimp_get_profile": (I, [P, P, I, C.POINTER(I)]),
+        "bimp_reset_profile": (I, [P]),
+        "bimp_set_flags": (I, [P, I]),
+        "bimp_device_count": (I, [C.POINTER(I)]),
+        "bimp_device_malloc": (I, [C.POINTER(P), SZ]),
+        "bimp_device_free": (I, [P]),
+        "bimp_memcpy_h2d": (I, [P, P, SZ]),
+        "bimp_memcpy_d2h": (I, [P, P, SZ]),
+        "bimp_stream_create": (I, [C.POINTER(P)]),
+        "bimp_stream_destroy": (I, [P]),
+        "bimp_stream_sync": (I, [P]),
+        "bimp_time_detect": (I, [P, P, SZ, P, I, I, P, C.POINTER(C.c_float)]),
+        "bimp_copy_kernel": (I, [P, P, SZ, P]),
+        "bimp_synth_frame": (I, [I, I, I, C.c_uint64, P, SZ]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(st, ctx=None):
+    if st < 0:
+        msg = lib.bimp_last_error(ctx).decode() if ctx else lib.bimp_status_string(st).decode()
+        raise BimpError(st, msg or lib.bimp_status_string(st).decode())
+    return st
+
+
+def default_options():
+    o = Options()
+    lib.bimp_default_options(C.byref(o))
+    return o
+
+
+def snap_transform_size(n):
+    """snapTransformSize (cuda_bimp.cu:200-219)."""
+    return _check(lib.bimp_snap_transform_size(int(n)))
+
+
+def make_lambdas_log(start, end, scales_per_octave):
+    """makeLambdasLog (cuda_bimp.cu:927-937): float-accumulated geometric wavelengths."""
+    n = _check(lib.bimp_make_lambdas_log(start, end, scales_per_octave, None, 0))
+    out = np.zeros(max(n, 1), np.float32)
+    lib.bimp_make_lambdas_log(start, end, scales_per_octave, _ptr(out), n)
+    return out[:n]
+
+
+def synth_frame(width, height, nshapes, seed):
+    """Seeded synthetic uint8 frame (rectangles/ellipses on a constant level, sigma-5 noise)."""
+    out = np.zeros((height, width), np.uint8)
+    _check(lib.bimp_synth_frame(width, height, nshapes, seed, _ptr(out), width))
+    return out
+
+
+def device_count():
+    n = C.c_int(0)
+    lib.bimp_device_count(C.byref(n))
+    return n.value
+
+
+class DeviceBuffer:
+    """Raw device allocation (plumbing for device-resident frames and outputs)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib.bimp_device_malloc(C.byref(p), self.nbytes))
+        self.ptr = p
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib.bimp_memcpy_h2d(self.ptr, _ptr(arr), arr.nbytes))
+
+    def download(self, arr):
+        assert arr.flags.c_contiguous and arr.nbytes <= self.nbytes
+        _check(lib.bimp_memcpy_d2h(_ptr(arr), self.ptr, arr.nbytes))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            lib.bimp_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        _check(lib.bimp_stream_create(C.byref(s)))
+        self.handle = s
+
+    def sync(self):
+        _check(lib.bimp_stream_sync(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.bimp_stream_destroy(self.handle)
+            self.handle = None
+
+
+class CudaStuff:
+    """Per-frame-size state: bank spectra and buffers (CudaStuff, cuda_bimp.cu:222-460)."""
+
+    def __init__(self, width, height, lambdas, options=None):
+        lam = np.ascontiguousarray(lambdas, np.float32)
+        ctx = C.c_void_p()
+        opt = options if options is not None else default_options()
+        _check(lib.bimp_create(int(width), int(height), _ptr(lam), lam.size, C.byref(opt),
+                               C.byref(ctx)))
+        self.ctx = ctx
+        self.width, self.height = int(width), int(height)
+        self.options = opt
+        self.ks = []
+        for i in range(lib.bimp_num_scales(ctx)):
+            si = ScaleInfo()
+            _check(lib.bimp_get_scale_info(ctx, i, C.byref(si)), ctx)
+            self.ks.append(si)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib.bimp_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        return _check(st, self.ctx)
+
+    # ---- stages
+    def filter(self, img, stream=None):
+        img = np.ascontiguousarray(img, np.uint8)
+        assert img.shape == (self.height, self.width)
+        self._chk(lib.bimp_filter_u8(self.ctx, _ptr(img), self.width, 0, stream))
+
+    def cells(self, stream=None):
+        self._chk(lib.bimp_cells(self.ctx, stream))
+
+    def keypoints(self, cap=KP_MAX_POINTS, stream=None):
+        out = np.zeros(max(cap, 1), KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        self._chk(lib.bimp_keypoints(self.ctx, _ptr(out), cap, 0, C.byref(n), stream))
+        return out[:min(n.value, cap)].copy(), n.value
+
+    def detect(self, img, cap=KP_MAX_POINTS, stream=None):
+        img = np.ascontiguousarray(img, np.uint8)
+        assert img.shape == (self.height, self.width)
+        out = np.zeros(max(cap, 1), KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        self._chk(lib.bimp_detect_u8(self.ctx, _ptr(img), self.width, 0, _ptr(out), cap, 0,
+                                     C.byref(n), stream))
+        return out[:min(n.value, cap)].copy(), n.value
+
+    def detect_matrix(self, img, cap=KP_MAX_POINTS, stream=None):
+        img = np.ascontiguousarray(img, np.uint8)
+        mat = np.zeros((KP_VECTOR_LENGTH, max(cap, 1)), np.float32)
+        n = C.c_int(0)
+        self._chk(lib.bimp_detect_u8_matrix(self.ctx, _ptr(img), self.width, 0, _ptr(mat),
+                                            mat.shape[1], cap, 0, C.byref(n), stream))
+        return mat[:, :min(n.value, cap)].copy(), n.value
+
+    def get_map(self, scale, which, orientation=0, stream=None):
+        si = self.ks[scale]
+        if which == MAP_COMPLEX:
+            shape = (si.map_h, si.map_w)
+        else:
+            shape = (si.data_h, si.data_w)
+        out = np.zeros(shape, np.float32)
+        self._chk(lib.bimp_get_map(self.ctx, scale, which, orientation, _ptr(out), shape[1], 0,
+                                   stream))
+        return out
+
+    def set_map(self, scale, which, arr, orientation=0, stream=None):
+        arr = np.ascontiguousarray(arr, np.float32)
+        self._chk(lib.bimp_set_map(self.ctx, scale, which, orientation, _ptr(arr),
+                                   arr.shape[1], 0, stream))
+
+    # ---- profiling
+    def set_flags(self, flags):
+        self._chk(lib.bimp_set_flags(self.ctx, flags))
+
+    def reset_profile(self):
+        self._chk(lib.bimp_reset_profile(self.ctx))
+
+    def profile(self):
+        buf = (ProfileEntry * 32)()
+        n = C.c_int(0)
+        self._chk(lib.bimp_get_profile(self.ctx, buf, 32, C.byref(n)))
+        return {buf[i].name.decode(): {"launches": buf[i].launches, "total_ms": buf[i].total_ms,
+                                       "bytes": buf[i].bytes} for i in range(n.value)}
+
+    def time_detect(self, d_img, pitch, d_out, cap, iters, stream):
+        ms = C.c_float(0)
+        self._chk(lib.bimp_time_detect(self.ctx, d_img.ptr, pitch, d_out.ptr, cap, iters,
+                                       stream.handle if stream else None, C.byref(ms)))
+        return ms.value
+
+
+def get_keypoints(img, cs, cap=KP_MAX_POINTS):
+    """bimp::cuda::getKeypoints (cuda_bimp.cu:109-140) + downloadKeypoints (utils.cpp:130-150):
+    returns the keypoint records (cv::KeyPoint field order) and the total count."""
+    return cs.detect(img, cap)
+
+
+def keypoints_to_matrix(kps):
+    """utils::uploadKeypoints layout (utils.cpp:109-128): 7 x N SURF_CUDA matrix."""
+    kps = np.ascontiguousarray(kps, KEYPOINT_DTYPE)
+    n = kps.size
+    mat = np.zeros((KP_VECTOR_LENGTH, max(n, 1)), np.float32)
+    _check(lib.bimp_keypoints_to_matrix(_ptr(kps), n, _ptr(mat), mat.shape[1]))
+    return mat[:, :n]
+
+
+def matrix_to_keypoints(mat):
+    """utils::downloadKeypoints layout (utils.cpp:130-150)."""
+    mat = np.ascontiguousarray(mat, np.float32)
+    n = mat.shape[1]
+    kps = np.zeros(max(n, 1), KEYPOINT_DTYPE)
+    _check(lib.bimp_matrix_to_keypoints(_ptr(mat), mat.shape[1], n, _ptr(kps)))
+    return kps[:n]
+
+
+class Context:
+    """bimp::Context (bimp.hpp:19-51, bimp.cpp:7-129) for grayscale uint8 frames.
+
+    The default bank is makeLambdasLog(4, 64, 2) as in the reference constructor
+    (bimp.cpp:8); ``lambdas`` overrides it (the reference's CudaStuff takes any bank)."""
+
+    def __init__(self, image, lambdas=None, options=None):
+        image = np.ascontiguousarray(image, np.uint8)
+        if image.ndim != 2:
+            raise ValueError("grayscale uint8 frames only")
+        self.lambdas = (make_lambdas_log(4, 64, 2) if lambdas is None
+                        else np.asarray(lambdas, np.float32))
+        self.cuda_data = CudaStuff(image.shape[1], image.shape[0], self.lambdas, options)
+        self.current_image = image
+        self.has_changed_keypoints = True
+        self._keypoints = None
+        self._count = 0
+
+    def width(self):
+        return self.current_image.shape[1]
+
+    def height(self):
+        return self.current_image.shape[0]
+
+    def loadImage(self, image):
+        image = np.ascontiguousarray(image, np.uint8)
+        if image.shape != self.current_image.shape:
+            raise ValueError("frame size differs from the context's")
+        self.current_image = image
+        self.has_changed_keypoints = True
+
+    def getKeypoints(self):
+        """7 x N SURF_CUDA keypoint matrix (the GpuMat the reference returns, host copy)."""
+        if self.has_changed_keypoints:
+            self._keypoints, self._count = self.cuda_data.detect(self.current_image)
+            self.has_changed_keypoints = False
+        return keypoints_to_matrix(self._keypoints)
+
+    def getKeypointsAndDownload(self):
+        if self.has_changed_keypoints:
+            self.getKeypoints()
+        return self._keypoints.copy()
+
+    def getResponses(self):
+        """Per-scale parameters and response maps (Context::getResponses, bimp.cpp:94-104)."""
+        out = []
+        for i, si in enumerate(self.cuda_data.ks):
+            out.append({"info": si.as_dict(),
+                        "double": self.cuda_data.get_map(i, MAP_DOUBLE),
+                        "single": self.cuda_data.get_map(i, MAP_SINGLE)})
+        return out
+
+    def setCPUMode(self, cpu_mode):
+        if cpu_mode:
+            raise NotImplementedError("the CPU-NMS mode of the reference is not provided")

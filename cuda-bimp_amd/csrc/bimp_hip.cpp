@@ -1,0 +1,762 @@
+// bimp_hip.cpp -- host context and C ABI of libbimp_hip.so (declared in include/bimp_hip.h).
+//
+// One context = the reference's CudaStuff (bimp/src/cuda_bimp.cu:222-460): per-scale
+// parameters, the quadrature filter bank resident in HBM, and every buffer one frame needs,
+// allocated once.  A frame is a fixed sequence of kernel launches on the caller's stream with
+// no host synchronisation until the keypoint count is read back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/bimp_hip.h"
+#include "bank.hpp"
+#include "kernels.hpp"
+
+using bimpk::kNumOri;
+using bimpk::kUniqueOri;
+
+namespace {
+
+enum Family {
+    F_INIT, F_COPY_IN, F_PYR, F_FWD_ROWS, F_FWD_COLS, F_INV_COLS, F_INV_ROWS, F_CELLS,
+    F_MAX, F_NMS_COUNT, F_NMS_SCAN, F_NMS_WRITE, F_COUNT
+};
+const char* kFamilyNames[F_COUNT] = {"init",     "copy_in",  "pyr_down",  "fwd_rows",
+                                     "fwd_cols", "inv_cols", "inv_rows",  "cells",
+                                     "max",      "nms_count", "nms_scan", "nms_write"};
+
+struct Scale {
+    bimph::ScaleGeom g;
+    int Hc = 0, Wc = 0, ldc = 0;
+    bimpk::CellParams cp;
+    float* G = nullptr;    // kUniqueOri x FW x FH
+    float2* DT = nullptr;  // (FW/2+1) x FH
+    float* C = nullptr;    // 8 x Hc x ldc
+    float* D = nullptr;    // H x W
+    float* S = nullptr;    // H x W
+};
+
+struct ProfEv {
+    hipEvent_t a, b;
+    int fam;
+    double bytes;
+};
+
+}  // namespace
+
+struct bimp_ctx {
+    int width = 0, height = 0;
+    bimp_options opt;
+    std::vector<Scale> sc;
+    int maxLevel = 0;
+    std::vector<uint8_t*> level;  // level[0] is the staging copy of a host frame
+    std::vector<int> levelW, levelH;
+    float2* R = nullptr;
+    float2* T = nullptr;
+    std::map<int, float2*> tw;
+    unsigned* maxslots = nullptr;
+    int* counter = nullptr;
+    int* blockCounts = nullptr;
+    int totalBlocks = 0;
+    bimpk::NmsScale* nmsTable = nullptr;
+    void* kpBuf = nullptr;
+    size_t kpBufBytes = 0;
+    bool maxValid = false;
+    std::string err;
+    // profiling
+    std::vector<ProfEv> ev;
+    size_t evUsed = 0;
+    double profMs[F_COUNT] = {0};
+    double profBytes[F_COUNT] = {0};
+    int profN[F_COUNT] = {0};
+    hipStream_t lastStream = nullptr;
+};
+
+namespace {
+
+int fail(bimp_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(bimp_ctx* c, hipError_t e, const char* what) {
+    return fail(c, BIMP_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                    \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc((void**)p, count * sizeof(T) + 256);
+}
+
+int round_half_away(double v) { return (int)std::round(v); }
+
+// Profiling brackets around one launch.
+struct LaunchScope {
+    bimp_ctx* c;
+    hipStream_t s;
+    ProfEv* e = nullptr;
+    LaunchScope(bimp_ctx* c_, hipStream_t s_, int fam, double bytes) : c(c_), s(s_) {
+        if (!(c->opt.flags & BIMP_FLAG_PROFILE)) return;
+        if (c->evUsed == c->ev.size()) {
+            ProfEv n;
+            (void)hipEventCreate(&n.a);
+            (void)hipEventCreate(&n.b);
+            c->ev.push_back(n);
+        }
+        e = &c->ev[c->evUsed++];
+        e->fam = fam;
+        e->bytes = bytes;
+        (void)hipEventRecord(e->a, s);
+    }
+    ~LaunchScope() {
+        if (e) (void)hipEventRecord(e->b, s);
+    }
+};
+
+void collect_profile(bimp_ctx* c) {
+    if (!c->evUsed) return;
+    (void)hipEventSynchronize(c->ev[c->evUsed - 1].b);
+    for (size_t i = 0; i < c->evUsed; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[i].a, c->ev[i].b) == hipSuccess) {
+            c->profMs[c->ev[i].fam] += ms;
+            c->profBytes[c->ev[i].fam] += c->ev[i].bytes;
+            c->profN[c->ev[i].fam] += 1;
+        }
+    }
+    c->evUsed = 0;
+}
+
+void free_all(bimp_ctx* c) {
+    for (auto& s : c->sc) {
+        (void)hipFree(s.G);
+        (void)hipFree(s.DT);
+        (void)hipFree(s.C);
+        (void)hipFree(s.D);
+        (void)hipFree(s.S);
+    }
+    for (auto* p : c->level) (void)hipFree(p);
+    for (auto& kv : c->tw) (void)hipFree(kv.second);
+    (void)hipFree(c->R);
+    (void)hipFree(c->T);
+    (void)hipFree(c->maxslots);
+    (void)hipFree(c->counter);
+    (void)hipFree(c->blockCounts);
+    (void)hipFree(c->nmsTable);
+    (void)hipFree(c->kpBuf);
+    for (auto& e : c->ev) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+}
+
+int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hipStream_t s) {
+    const int W0 = c->width, H0 = c->height;
+    {
+        LaunchScope ls(c, s, F_INIT, 0);
+        HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
+    }
+    c->maxValid = false;
+    const uint8_t* src0 = img;
+    size_t pitch0 = pitch;
+    if (!on_device) {
+        LaunchScope ls(c, s, F_COPY_IN, 2.0 * W0 * H0);
+        HIPCHK(c, hipMemcpy2DAsync(c->level[0], W0, img, pitch, W0, H0, hipMemcpyHostToDevice, s));
+        src0 = c->level[0];
+        pitch0 = W0;
+    }
+    std::vector<const uint8_t*> lv(c->maxLevel + 1);
+    std::vector<size_t> lp(c->maxLevel + 1);
+    lv[0] = src0;
+    lp[0] = pitch0;
+    for (int l = 1; l <= c->maxLevel; ++l) {
+        LaunchScope ls(c, s, F_PYR,
+                       (double)c->levelW[l - 1] * c->levelH[l - 1] + (double)c->levelW[l] * c->levelH[l]);
+        HIPCHK(c, bimpk::launch_pyr_down(lv[l - 1], c->levelW[l - 1], c->levelH[l - 1], lp[l - 1],
+                                         c->level[l], c->levelW[l], c->levelH[l],
+                                         (size_t)c->levelW[l], s));
+        lv[l] = c->level[l];
+        lp[l] = (size_t)c->levelW[l];
+    }
+    for (auto& sc : c->sc) {
+        const auto& g = sc.g;
+        const int ncols = g.FW / 2 + 1;
+        const float2* twW = c->tw[g.FW];
+        const float2* twH = c->tw[g.FH];
+        {
+            LaunchScope ls(c, s, F_FWD_ROWS, (double)g.H * g.W + 8.0 * g.H * ncols);
+            HIPCHK(c, bimpk::launch_fwd_rows(g.FW, lv[g.level], lp[g.level], g.H, g.W, g.kX,
+                                             c->opt.input_scale, c->R, ncols, twW, s));
+        }
+        {
+            LaunchScope ls(c, s, F_FWD_COLS, 8.0 * g.H * ncols + 8.0 * ncols * g.FH);
+            HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, g.H, g.kX, ncols, sc.DT, twH, s));
+        }
+        {
+            LaunchScope ls(c, s, F_INV_COLS,
+                           8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
+                               8.0 * kNumOri * sc.Hc * g.FW);
+            HIPCHK(c, bimpk::launch_inv_cols(g.FH, sc.DT, sc.G, g.FW, sc.Hc, c->T, twH, s));
+        }
+        {
+            LaunchScope ls(c, s, F_INV_ROWS,
+                           8.0 * kNumOri * sc.Hc * g.FW + 4.0 * kNumOri * sc.Hc * sc.Wc);
+            HIPCHK(c, bimpk::launch_inv_rows(g.FW, c->T, sc.Hc, sc.Wc, sc.C, sc.ldc, twW, s));
+        }
+    }
+    return BIMP_OK;
+}
+
+int run_cells(bimp_ctx* c, hipStream_t s) {
+    for (size_t i = 0; i < c->sc.size(); ++i) {
+        auto& sc = c->sc[i];
+        const auto& g = sc.g;
+        LaunchScope ls(c, s, F_CELLS,
+                       4.0 * kNumOri * (g.H + 1.0) * (g.W + 1.0) + 8.0 * g.H * g.W);
+        HIPCHK(c, bimpk::launch_cells(sc.C, sc.Hc, sc.ldc, g.FH, g.FW, g.H, g.W, sc.cp, sc.D, sc.S,
+                                      c->maxslots + i, s));
+    }
+    c->maxValid = true;
+    return BIMP_OK;
+}
+
+int run_keypoints(bimp_ctx* c, void* out, int cap, int matrix_ld, hipStream_t s) {
+    if (!c->maxValid) {
+        {
+            LaunchScope ls(c, s, F_INIT, 0);
+            HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
+        }
+        for (size_t i = 0; i < c->sc.size(); ++i) {
+            auto& sc = c->sc[i];
+            LaunchScope ls(c, s, F_MAX, 4.0 * sc.g.H * sc.g.W);
+            HIPCHK(c, bimpk::launch_max(sc.D, sc.g.H * sc.g.W, c->maxslots + i, s));
+        }
+        c->maxValid = true;
+    }
+    double dbytes = 0;
+    for (auto& sc : c->sc) dbytes += 4.0 * sc.g.H * sc.g.W;
+    {
+        LaunchScope ls(c, s, F_NMS_COUNT, dbytes);
+        HIPCHK(c, bimpk::launch_nms_count(c->nmsTable, (int)c->sc.size(), c->totalBlocks,
+                                          c->maxslots, c->opt.nms_threshold, c->blockCounts, s));
+    }
+    {
+        LaunchScope ls(c, s, F_NMS_SCAN, 8.0 * c->totalBlocks);
+        HIPCHK(c, bimpk::launch_nms_scan(c->blockCounts, c->totalBlocks, c->counter, s));
+    }
+    {
+        LaunchScope ls(c, s, F_NMS_WRITE, dbytes);
+        HIPCHK(c, bimpk::launch_nms_write(c->nmsTable, (int)c->sc.size(), c->totalBlocks,
+                                          c->maxslots, c->opt.nms_threshold, c->blockCounts,
+                                          out, cap, matrix_ld, s));
+    }
+    return BIMP_OK;
+}
+
+int ensure_kp_buf(bimp_ctx* c, size_t bytes) {
+    if (c->kpBufBytes >= bytes) return BIMP_OK;
+    (void)hipFree(c->kpBuf);
+    c->kpBuf = nullptr;
+    c->kpBufBytes = 0;
+    HIPCHK(c, hipMalloc(&c->kpBuf, bytes));
+    c->kpBufBytes = bytes;
+    return BIMP_OK;
+}
+
+// Runs NMS into `out` (host or device) and reads the count back.
+int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_device, int* count,
+                  hipStream_t s) {
+    const size_t elem = matrix_ld > 0 ? 0 : sizeof(bimp_keypoint);
+    void* target = out;
+    if (!out_on_device && cap > 0) {
+        const size_t bytes = matrix_ld > 0 ? (size_t)BIMP_KP_VECTOR_LENGTH * matrix_ld * 4
+                                           : (size_t)cap * elem;
+        int st = ensure_kp_buf(c, bytes);
+        if (st) return st;
+        target = c->kpBuf;
+    }
+    int st = run_keypoints(c, target, cap, matrix_ld, s);
+    if (st) return st;
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->counter, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const int n = std::min(total, cap);
+    if (!out_on_device && n > 0) {
+        if (matrix_ld > 0) {
+            HIPCHK(c, hipMemcpy2D(out, (size_t)matrix_ld * 4, target, (size_t)matrix_ld * 4,
+                                  (size_t)n * 4, BIMP_KP_VECTOR_LENGTH, hipMemcpyDeviceToHost));
+        } else {
+            HIPCHK(c, hipMemcpy(out, target, (size_t)n * elem, hipMemcpyDeviceToHost));
+        }
+    }
+    if (count) *count = total;
+    if (c->opt.flags & BIMP_FLAG_PROFILE) collect_profile(c);
+    if (total > cap) {
+        c->err = "keypoints (" + std::to_string(total) + ") exceed the output capacity (" +
+                 std::to_string(cap) + ")";
+        return BIMP_W_OVERFLOW;
+    }
+    return BIMP_OK;
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+void bimp_default_options(bimp_options* o) {
+    if (!o) return;
+    o->gamma = 0.5f;
+    o->sigma_ratio = 0.56f;
+    o->support = 7.0f;
+    o->pyr_threshold = 7.5f;
+    o->nms_threshold = 0.104;
+    o->input_scale = (float)(255.0 / 65536.0);
+    o->flags = 0;
+}
+
+int bimp_snap_transform_size(int n) { return n > 0 ? bimph::snap_transform_size(n) : BIMP_E_ARG; }
+
+int bimp_make_lambdas_log(float start, float end, int spo, float* out, int cap) {
+    if (spo <= 0 || !(start > 0.f) || cap < 0) return BIMP_E_ARG;
+    const float multiplier = (float)std::pow(2., 1. / spo);
+    int n = 0;
+    for (float lambda = start; (double)lambda <= (double)end * 1.05; lambda *= multiplier) {
+        if (out && n < cap) out[n] = lambda;
+        if (++n > 1000000) break;
+    }
+    return n;
+}
+
+int bimp_create(int width, int height, const float* lambdas, int n, const bimp_options* opt,
+                bimp_ctx** out) {
+    if (!out) return BIMP_E_ARG;
+    *out = nullptr;
+    if (width < 4 || height < 4 || !lambdas || n <= 0 || n > 64) return BIMP_E_ARG;
+    bimp_ctx* c = new bimp_ctx;
+    c->width = width;
+    c->height = height;
+    if (opt)
+        c->opt = *opt;
+    else
+        bimp_default_options(&c->opt);
+    std::vector<float> lam(lambdas, lambdas + n);
+    std::sort(lam.begin(), lam.end());  // cuda_bimp.cu:224
+    for (float l : lam) {
+        Scale s;
+        if (!bimph::derive_scale(l, width, height, c->opt.pyr_threshold, c->opt.support, &s.g) ||
+            s.g.lambda < 1.0f || s.g.H < 4 || s.g.W < 4 || s.g.kW < 1) {
+            delete c;
+            return BIMP_E_ARG;
+        }
+        if (!bimpk::fft_size_supported(s.g.FH) || !bimpk::fft_size_supported(s.g.FW) ||
+            s.g.FW % bimpk::col_batch(s.g.FH) != 0) {
+            delete c;
+            return BIMP_E_UNSUPPORTED;
+        }
+        const auto& g = s.g;
+        s.Wc = std::min(g.FW, g.W + 1);
+        s.Hc = std::min(g.FH, g.H + (g.W == g.FW ? 2 : 1));
+        s.ldc = (s.Wc + 31) / 32 * 32;
+        for (int o = 0; o < kNumOri; ++o) {
+            const double theta = (double)(float)(o * 3.1415926535897932384626433832795 / kNumOri);
+            s.cp.ea[o] = round_half_away(0.6 * g.lambda * std::cos(theta));
+            s.cp.eb[o] = round_half_away(0.6 * g.lambda * std::sin(theta));
+            s.cp.ia[o] = round_half_away(0.6 * g.lambda * std::sin(theta));
+            s.cp.ib[o] = round_half_away(0.6 * g.lambda * std::cos(theta));
+            s.cp.ra[o] = round_half_away(0.5 * s.cp.ia[o]);
+            s.cp.rb[o] = round_half_away(0.5 * s.cp.ib[o]);
+        }
+        c->sc.push_back(s);
+    }
+    int st = BIMP_OK;
+    auto bail = [&](int code, const std::string& msg) {
+        free_all(c);
+        delete c;
+        (void)msg;
+        return code;
+    };
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bail(BIMP_E_DEVICE, "no device");
+    // pyramid levels
+    for (auto& s : c->sc) c->maxLevel = std::max(c->maxLevel, s.g.level);
+    c->levelW.resize(c->maxLevel + 1);
+    c->levelH.resize(c->maxLevel + 1);
+    c->level.assign(c->maxLevel + 1, nullptr);
+    c->levelW[0] = width;
+    c->levelH[0] = height;
+    for (int l = 1; l <= c->maxLevel; ++l) {
+        c->levelW[l] = (c->levelW[l - 1] + 1) / 2;
+        c->levelH[l] = (c->levelH[l - 1] + 1) / 2;
+    }
+    for (int l = 0; l <= c->maxLevel; ++l)
+        if (dalloc(&c->level[l], (size_t)c->levelW[l] * c->levelH[l]) != hipSuccess)
+            return bail(BIMP_E_ALLOC, "level");
+    size_t rMax = 0, tMax = 0;
+    std::vector<int> sizes;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int threads = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+    for (auto& s : c->sc) {
+        const auto& g = s.g;
+        const int ncols = g.FW / 2 + 1;
+        rMax = std::max(rMax, (size_t)g.H * ncols);
+        tMax = std::max(tMax, (size_t)kNumOri * s.Hc * g.FW);
+        sizes.push_back(g.FH);
+        sizes.push_back(g.FW);
+        if (dalloc(&s.G, (size_t)kUniqueOri * g.FW * g.FH) != hipSuccess ||
+            dalloc(&s.DT, (size_t)ncols * g.FH) != hipSuccess ||
+            dalloc(&s.C, (size_t)kNumOri * s.Hc * s.ldc) != hipSuccess ||
+            dalloc(&s.D, (size_t)g.H * g.W) != hipSuccess ||
+            dalloc(&s.S, (size_t)g.H * g.W) != hipSuccess)
+            return bail(BIMP_E_ALLOC, "scale buffers");
+        std::vector<float> bank;
+        bimph::quadrature_bank(g, c->opt.gamma, c->opt.sigma_ratio, threads, bank);
+        if (hipMemcpy(s.G, bank.data(), bank.size() * sizeof(float), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return bail(BIMP_E_DEVICE, "bank upload");
+        // complex-cell maps start at zero so the unused padding columns are defined
+        if (hipMemset(s.C, 0, (size_t)kNumOri * s.Hc * s.ldc * sizeof(float)) != hipSuccess)
+            return bail(BIMP_E_DEVICE, "memset");
+    }
+    if (dalloc(&c->R, rMax) != hipSuccess || dalloc(&c->T, tMax) != hipSuccess)
+        return bail(BIMP_E_ALLOC, "scratch");
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    for (int n2 : sizes) {
+        if (bimpk::prepare_size(n2) != hipSuccess) return bail(BIMP_E_DEVICE, "kernel attributes");
+        std::vector<float2> t(n2);
+        for (int k = 0; k < n2; ++k) {
+            const double a = -2.0 * 3.1415926535897932384626433832795 * k / n2;
+            t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        float2* d = nullptr;
+        if (dalloc(&d, n2) != hipSuccess) return bail(BIMP_E_ALLOC, "twiddles");
+        c->tw[n2] = d;
+        if (hipMemcpy(d, t.data(), n2 * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(BIMP_E_DEVICE, "twiddle upload");
+    }
+    // NMS descriptors
+    std::vector<bimpk::NmsScale> tab;
+    int blocks = 0;
+    for (size_t i = 0; i < c->sc.size(); ++i) {
+        const auto& s = c->sc[i];
+        bimpk::NmsScale d;
+        d.D = s.D;
+        d.H = s.g.H;
+        d.W = s.g.W;
+        d.off = s.g.off;
+        d.block0 = blocks;
+        d.nblocks = (s.g.H * s.g.W + bimpk::kNmsBlockPixels - 1) / bimpk::kNmsBlockPixels;
+        d.lambda = s.g.lambda;
+        d.pyrstep = s.g.pyrstep;
+        d.slot = (int)i;
+        blocks += d.nblocks;
+        tab.push_back(d);
+    }
+    c->totalBlocks = blocks;
+    if (dalloc(&c->nmsTable, tab.size()) != hipSuccess || dalloc(&c->blockCounts, blocks) != hipSuccess ||
+        dalloc(&c->maxslots, c->sc.size()) != hipSuccess || dalloc(&c->counter, 1) != hipSuccess)
+        return bail(BIMP_E_ALLOC, "nms buffers");
+    if (hipMemcpy(c->nmsTable, tab.data(), tab.size() * sizeof(bimpk::NmsScale),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return bail(BIMP_E_DEVICE, "nms table upload");
+    (void)st;
+    *out = c;
+    return BIMP_OK;
+}
+
+int bimp_destroy(bimp_ctx* c) {
+    if (!c) return BIMP_E_ARG;
+    (void)hipDeviceSynchronize();
+    free_all(c);
+    delete c;
+    return BIMP_OK;
+}
+
+int bimp_num_scales(const bimp_ctx* c) { return c ? (int)c->sc.size() : BIMP_E_ARG; }
+
+int bimp_get_scale_info(const bimp_ctx* c, int i, bimp_scale_info* o) {
+    if (!c || !o || i < 0 || i >= (int)c->sc.size()) return BIMP_E_ARG;
+    const auto& s = c->sc[i];
+    o->lambda2 = s.g.lambda2;
+    o->lambda = s.g.lambda;
+    o->pyrstep = s.g.pyrstep;
+    o->kernel_size = s.g.kW;
+    o->kernel_centre = s.g.kX;
+    o->border = s.g.off;
+    o->level = s.g.level;
+    o->data_h = s.g.H;
+    o->data_w = s.g.W;
+    o->fft_h = s.g.FH;
+    o->fft_w = s.g.FW;
+    o->map_h = s.Hc;
+    o->map_w = s.Wc;
+    o->map_pitch = s.ldc;
+    return BIMP_OK;
+}
+
+int bimp_filter_u8(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, bimp_stream_t stream) {
+    if (!c || !img || pitch < (size_t)c->width) return fail(c, BIMP_E_ARG, "bad image arguments");
+    hipStream_t s = (hipStream_t)stream;
+    c->lastStream = s;
+    return run_filter(c, img, pitch, on_device, s);
+}
+
+int bimp_cells(bimp_ctx* c, bimp_stream_t stream) {
+    if (!c) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    c->lastStream = s;
+    return run_cells(c, s);
+}
+
+int bimp_keypoints(bimp_ctx* c, bimp_keypoint* out, int cap, int out_on_device, int* count,
+                   bimp_stream_t stream) {
+    if (!c || cap < 0 || (cap > 0 && !out)) return fail(c, BIMP_E_ARG, "bad output arguments");
+    hipStream_t s = (hipStream_t)stream;
+    c->lastStream = s;
+    return keypoints_out(c, out, cap, 0, out_on_device, count, s);
+}
+
+int bimp_detect_u8(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_device,
+                   bimp_keypoint* out, int cap, int out_on_device, int* count, bimp_stream_t stream) {
+    if (!c || !img || pitch < (size_t)c->width || cap < 0 || (cap > 0 && !out))
+        return fail(c, BIMP_E_ARG, "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    c->lastStream = s;
+    int st = run_filter(c, img, pitch, img_on_device, s);
+    if (st) return st;
+    st = run_cells(c, s);
+    if (st) return st;
+    return keypoints_out(c, out, cap, 0, out_on_device, count, s);
+}
+
+int bimp_detect_u8_matrix(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_device,
+                          float* mat, int ld, int cap, int mat_on_device, int* count,
+                          bimp_stream_t stream) {
+    if (!c || !img || pitch < (size_t)c->width || cap < 0 || ld < cap || ld <= 0 || !mat)
+        return fail(c, BIMP_E_ARG, "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    c->lastStream = s;
+    int st = run_filter(c, img, pitch, img_on_device, s);
+    if (st) return st;
+    st = run_cells(c, s);
+    if (st) return st;
+    return keypoints_out(c, mat, cap, ld, mat_on_device, count, s);
+}
+
+int bimp_get_map(bimp_ctx* c, int i, int which, int o, float* dst, size_t pitch, int dst_on_device,
+                 bimp_stream_t stream) {
+    if (!c || !dst || i < 0 || i >= (int)c->sc.size()) return BIMP_E_ARG;
+    const auto& s = c->sc[i];
+    hipStream_t st = (hipStream_t)stream;
+    const float* src;
+    int rows, cols;
+    size_t spitch;
+    if (which == BIMP_MAP_DOUBLE || which == BIMP_MAP_SINGLE) {
+        src = which == BIMP_MAP_DOUBLE ? s.D : s.S;
+        rows = s.g.H;
+        cols = s.g.W;
+        spitch = s.g.W;
+    } else if (which == BIMP_MAP_COMPLEX && o >= 0 && o < kNumOri) {
+        src = s.C + (size_t)o * s.Hc * s.ldc;
+        rows = s.Hc;
+        cols = s.Wc;
+        spitch = s.ldc;
+    } else {
+        return fail(c, BIMP_E_ARG, "bad map selector");
+    }
+    if (pitch < (size_t)cols) return fail(c, BIMP_E_ARG, "pitch too small");
+    HIPCHK(c, hipMemcpy2DAsync(dst, pitch * 4, src, spitch * 4, (size_t)cols * 4, rows,
+                               dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return BIMP_OK;
+}
+
+int bimp_set_map(bimp_ctx* c, int i, int which, int o, const float* src, size_t pitch,
+                 int src_on_device, bimp_stream_t stream) {
+    if (!c || !src || i < 0 || i >= (int)c->sc.size()) return BIMP_E_ARG;
+    auto& s = c->sc[i];
+    hipStream_t st = (hipStream_t)stream;
+    float* dst;
+    int rows, cols;
+    size_t dpitch;
+    if (which == BIMP_MAP_DOUBLE || which == BIMP_MAP_SINGLE) {
+        dst = which == BIMP_MAP_DOUBLE ? s.D : s.S;
+        rows = s.g.H;
+        cols = s.g.W;
+        dpitch = s.g.W;
+        if (which == BIMP_MAP_DOUBLE) c->maxValid = false;
+    } else if (which == BIMP_MAP_COMPLEX && o >= 0 && o < kNumOri) {
+        dst = s.C + (size_t)o * s.Hc * s.ldc;
+        rows = s.Hc;
+        cols = s.Wc;
+        dpitch = s.ldc;
+    } else {
+        return fail(c, BIMP_E_ARG, "bad map selector");
+    }
+    if (pitch < (size_t)cols) return fail(c, BIMP_E_ARG, "pitch too small");
+    HIPCHK(c, hipMemcpy2DAsync(dst, dpitch * 4, src, pitch * 4, (size_t)cols * 4, rows,
+                               src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return BIMP_OK;
+}
+
+int bimp_keypoints_to_matrix(const bimp_keypoint* k, int n, float* mat, int ld) {
+    if (n < 0 || ld < n || (n > 0 && (!k || !mat))) return BIMP_E_ARG;
+    for (int j = 0; j < n; ++j) {
+        mat[0 * (size_t)ld + j] = k[j].x;
+        mat[1 * (size_t)ld + j] = k[j].y;
+        const int one = 1, oct = k[j].octave;
+        std::memcpy(&mat[2 * (size_t)ld + j], &one, 4);
+        std::memcpy(&mat[3 * (size_t)ld + j], &oct, 4);
+        mat[4 * (size_t)ld + j] = k[j].size;
+        mat[5 * (size_t)ld + j] = k[j].angle;
+        mat[6 * (size_t)ld + j] = k[j].response;
+    }
+    return BIMP_OK;
+}
+
+int bimp_matrix_to_keypoints(const float* mat, int ld, int n, bimp_keypoint* k) {
+    if (n < 0 || ld < n || (n > 0 && (!k || !mat))) return BIMP_E_ARG;
+    for (int j = 0; j < n; ++j) {
+        k[j].x = mat[0 * (size_t)ld + j];
+        k[j].y = mat[1 * (size_t)ld + j];
+        std::memcpy(&k[j].class_id, &mat[2 * (size_t)ld + j], 4);
+        std::memcpy(&k[j].octave, &mat[3 * (size_t)ld + j], 4);
+        k[j].size = mat[4 * (size_t)ld + j];
+        k[j].angle = mat[5 * (size_t)ld + j];
+        k[j].response = mat[6 * (size_t)ld + j];
+    }
+    return BIMP_OK;
+}
+
+const char* bimp_last_error(const bimp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+const char* bimp_status_string(int s) {
+    switch (s) {
+        case BIMP_OK: return "ok";
+        case BIMP_W_OVERFLOW: return "keypoint overflow";
+        case BIMP_E_ARG: return "invalid argument";
+        case BIMP_E_ALLOC: return "allocation failed";
+        case BIMP_E_DEVICE: return "device error";
+        case BIMP_E_UNSUPPORTED: return "unsupported transform size";
+        default: return "unknown status";
+    }
+}
+
+int bimp_get_profile(bimp_ctx* c, bimp_profile_entry* out, int cap, int* n) {
+    if (!c || !n) return BIMP_E_ARG;
+    collect_profile(c);
+    int k = 0;
+    for (int f = 0; f < F_COUNT; ++f) {
+        if (!c->profN[f]) continue;
+        if (out && k < cap) {
+            std::snprintf(out[k].name, sizeof(out[k].name), "%s", kFamilyNames[f]);
+            out[k].launches = c->profN[f];
+            out[k].total_ms = c->profMs[f];
+            out[k].bytes = c->profBytes[f];
+        }
+        ++k;
+    }
+    *n = k;
+    return BIMP_OK;
+}
+
+int bimp_reset_profile(bimp_ctx* c) {
+    if (!c) return BIMP_E_ARG;
+    collect_profile(c);
+    for (int f = 0; f < F_COUNT; ++f) {
+        c->profMs[f] = 0;
+        c->profBytes[f] = 0;
+        c->profN[f] = 0;
+    }
+    return BIMP_OK;
+}
+
+int bimp_set_flags(bimp_ctx* c, int flags) {
+    if (!c) return BIMP_E_ARG;
+    collect_profile(c);
+    c->opt.flags = flags;
+    return BIMP_OK;
+}
+
+int bimp_device_count(int* n) {
+    if (!n) return BIMP_E_ARG;
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+    *n = k;
+    return BIMP_OK;
+}
+
+int bimp_device_malloc(void** p, size_t bytes) {
+    if (!p) return BIMP_E_ARG;
+    return hipMalloc(p, bytes ? bytes : 1) == hipSuccess ? BIMP_OK : BIMP_E_ALLOC;
+}
+int bimp_device_free(void* p) { return hipFree(p) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE; }
+int bimp_memcpy_h2d(void* d, const void* s, size_t b) {
+    return hipMemcpy(d, s, b, hipMemcpyHostToDevice) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+int bimp_memcpy_d2h(void* d, const void* s, size_t b) {
+    return hipMemcpy(d, s, b, hipMemcpyDeviceToHost) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+int bimp_stream_create(bimp_stream_t* s) {
+    if (!s) return BIMP_E_ARG;
+    hipStream_t h;
+    if (hipStreamCreate(&h) != hipSuccess) return BIMP_E_DEVICE;
+    *s = (bimp_stream_t)h;
+    return BIMP_OK;
+}
+int bimp_stream_destroy(bimp_stream_t s) {
+    return hipStreamDestroy((hipStream_t)s) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+int bimp_stream_sync(bimp_stream_t s) {
+    return hipStreamSynchronize((hipStream_t)s) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+
+int bimp_time_detect(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypoint* d_out,
+                     int cap, int iters, bimp_stream_t stream, float* ms) {
+    if (!c || !d_img || !d_out || iters <= 0 || !ms) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipEventRecord(a, s));
+    for (int it = 0; it < iters; ++it) {
+        int st = run_filter(c, d_img, pitch, 1, s);
+        if (!st) st = run_cells(c, s);
+        if (!st) st = run_keypoints(c, d_out, cap, 0, s);
+        if (st) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            return st;
+        }
+    }
+    HIPCHK(c, hipEventRecord(b, s));
+    HIPCHK(c, hipEventSynchronize(b));
+    HIPCHK(c, hipEventElapsedTime(ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (c->opt.flags & BIMP_FLAG_PROFILE) collect_profile(c);
+    return BIMP_OK;
+}
+
+int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t stream) {
+    if (!dst || !src) return BIMP_E_ARG;
+    if (bimpk::launch_copy(dst, src, bytes, (hipStream_t)stream) != hipSuccess) return BIMP_E_DEVICE;
+    return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+// fft_device.hpp -- register-resident Stockham FFT for gfx950 wavefronts.
+//
+// A length-N transform is owned by a group of TPV lanes (TPV = 64 for N >= 512: one
+// wavefront per vector); every lane keeps E = N/TPV complex values in registers.  Each
+// radix-R stage does E/R butterflies per lane entirely in registers; between stages the
+// group exchanges data through its own LDS region (N float2), so the only cross-lane
+// traffic is one LDS write + read per stage boundary.
+//
+// Index maps (Stockham autosort, radix list R0..RL, Ns = product of earlier radices):
+//   stage input  of butterfly j = lane + TPV*m, leg r : j + r*N/R
+//   stage output of butterfly j, leg r              : (j/Ns)*Ns*R + j%Ns + r*Ns
+// so the first stage reads element in_index(e) and the last stage produces element
+// out_index(e) in register slot e -- both are unit-stride across lanes, which makes global
+// loads and stores of a vector coalesced without any extra shuffle.
+//
+// Twiddles come from a per-N table tw[k] = exp(-2*pi*i*k/N) computed in FP64 on the host
+// and rounded to FP32; the inverse transform uses their conjugates (unnormalised).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace bimpdev {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- small DFTs
+// out[q] = sum_r in[r] * w^(q*r), w = exp(-+2*pi*i/R), natural output order, in place.
+template <bool INV>
+__device__ __forceinline__ void dft2(float2* v) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft3(float2* v) {
+    const float kS3 = 0.86602540378443865f;  // sin(pi/3)
+    const float2 a = v[0], b = v[1], c = v[2];
+    const float2 t1 = cadd(b, c);
+    const float2 t2 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+    const float2 t3 = cscale(csub(b, c), kS3);
+    v[0] = cadd(a, t1);
+    v[1] = cadd(t2, mul_mi<INV>(t3));
+    v[2] = csub(t2, mul_mi<INV>(t3));
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2* v) {
+    const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const float2 t2 = cadd(v[1], v[3]);
+    const float2 t3 = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2* v) {
+    const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+    const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+    const float2 a = v[0];
+    const float2 t1 = cadd(v[1], v[4]), t3 = csub(v[1], v[4]);
+    const float2 t2 = cadd(v[2], v[3]), t4 = csub(v[2], v[3]);
+    const float2 A1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    const float2 A2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    const float2 B1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const float2 B2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    v[0] = cadd(a, cadd(t1, t2));
+    v[1] = cadd(A1, mul_mi<INV>(B1));
+    v[4] = csub(A1, mul_mi<INV>(B1));
+    v[2] = cadd(A2, mul_mi<INV>(B2));
+    v[3] = csub(A2, mul_mi<INV>(B2));
+}
+
+// w8^q * x for q = 1..3 (w8 = exp(-+i*pi/4))
+template <bool INV, int Q>
+__device__ __forceinline__ float2 w8(float2 x) {
+    const float h = 0.70710678118654752f;
+    if (Q == 1) return INV ? make_float2(h * (x.x - x.y), h * (x.x + x.y))
+                           : make_float2(h * (x.x + x.y), h * (x.y - x.x));
+    if (Q == 2) return mul_mi<INV>(x);
+    return INV ? make_float2(-h * (x.x + x.y), h * (x.x - x.y))
+               : make_float2(h * (x.y - x.x), -h * (x.x + x.y));
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* v) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<INV>(e);
+    dft4<INV>(o);
+    o[1] = w8<INV, 1>(o[1]);
+    o[2] = w8<INV, 2>(o[2]);
+    o[3] = w8<INV, 3>(o[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[q] = cadd(e[q], o[q]);
+        v[q + 4] = csub(e[q], o[q]);
+    }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft16(float2* v) {
+    float2 e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft8<INV>(e);
+    dft8<INV>(o);
+    // w16^q, q = 1..7
+    const float c[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f,
+                        0.0f, -0.38268343236508978f, -0.70710678118654752f, -0.92387953251128674f};
+    const float s[8] = {0.0f, 0.38268343236508978f, 0.70710678118654752f, 0.92387953251128674f,
+                        1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f};
+#pragma unroll
+    for (int q = 1; q < 8; ++q) {
+        const float2 w = make_float2(c[q], INV ? s[q] : -s[q]);
+        o[q] = cmul(o[q], w);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        v[q] = cadd(e[q], o[q]);
+        v[q + 8] = csub(e[q], o[q]);
+    }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v) {
+    if constexpr (R == 2) dft2<INV>(v);
+    else if constexpr (R == 3) dft3<INV>(v);
+    else if constexpr (R == 4) dft4<INV>(v);
+    else if constexpr (R == 5) dft5<INV>(v);
+    else if constexpr (R == 8) dft8<INV>(v);
+    else if constexpr (R == 16) dft16<INV>(v);
+    else static_assert(R == 2, "unsupported radix");
+}
+
+// ---------------------------------------------------------------- plans
+template <int... Rs>
+struct RadList {};
+
+template <int N>
+struct FFTCfg;
+
+#define BIMP_FFT_CFG(N_, TPV_, ...)                               \
+    template <>                                                    \
+    struct FFTCfg<N_> {                                            \
+        static constexpr int N = N_;                               \
+        static constexpr int TPV = TPV_;                           \
+        static constexpr int E = N_ / TPV_;                        \
+        using Rad = RadList<__VA_ARGS__>;                          \
+    };
+BIMP_FFT_CFG(16, 4, 4, 4)
+BIMP_FFT_CFG(32, 8, 4, 4, 2)
+BIMP_FFT_CFG(64, 16, 4, 4, 4)
+BIMP_FFT_CFG(128, 16, 8, 8, 2)
+BIMP_FFT_CFG(256, 32, 8, 8, 4)
+BIMP_FFT_CFG(512, 64, 8, 8, 8)
+BIMP_FFT_CFG(1024, 64, 16, 16, 4)
+BIMP_FFT_CFG(1536, 64, 8, 8, 8, 3)
+BIMP_FFT_CFG(2048, 64, 16, 16, 8)
+BIMP_FFT_CFG(2560, 64, 8, 8, 8, 5)
+BIMP_FFT_CFG(4096, 64, 16, 16, 16)
+#undef BIMP_FFT_CFG
+
+template <class L>
+struct RadInfo;
+template <int R0, int... Rs>
+struct RadInfo<RadList<R0, Rs...>> {
+    static constexpr int first = R0;
+    static constexpr int last = RadInfo<RadList<Rs...>>::last;
+};
+template <int R0>
+struct RadInfo<RadList<R0>> {
+    static constexpr int first = R0;
+    static constexpr int last = R0;
+};
+
+// element index held in register slot e before the first stage / after the last stage
+template <int N>
+__device__ __forceinline__ int in_index(int e, int lane) {
+    using C = FFTCfg<N>;
+    constexpr int R = RadInfo<typename C::Rad>::first;
+    return lane + C::TPV * (e / R) + (e % R) * (N / R);
+}
+template <int N>
+__device__ __forceinline__ int out_index(int e, int lane) {
+    using C = FFTCfg<N>;
+    constexpr int R = RadInfo<typename C::Rad>::last;
+    return lane + C::TPV * (e / R) + (e % R) * (N / R);
+}
+
+// LDS ordering between the stages of one group.  Groups never share LDS regions, but a
+// workgroup barrier keeps the code independent of wave-level ordering assumptions.
+__device__ __forceinline__ void stage_sync() { __syncthreads(); }
+
+template <int N, bool INV, int Ns, int R>
+__device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
+                                            const float2* __restrict__ tw) {
+    using C = FFTCfg<N>;
+#pragma unroll
+    for (int m = 0; m < C::E / R; ++m) {
+        if constexpr (Ns > 1) {
+            const int j = lane + C::TPV * m;
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[r * k * (N / (Ns * R))];
+                if (INV) w.y = -w.y;
+                v[m * R + r] = cmul(v[m * R + r], w);
+            }
+        }
+        dft<R, INV>(&v[m * R]);
+    }
+}
+
+template <int N, int Ns, int R, int R2>
+__device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float2* lds, int lane) {
+    using C = FFTCfg<N>;
+#pragma unroll
+    for (int m = 0; m < C::E / R; ++m) {
+        const int j = lane + C::TPV * m;
+        const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[base + r * Ns] = v[m * R + r];
+    }
+    stage_sync();
+#pragma unroll
+    for (int m = 0; m < C::E / R2; ++m) {
+        const int j = lane + C::TPV * m;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[m * R2 + r] = lds[j + r * (N / R2)];
+    }
+    stage_sync();
+}
+
+template <int N, bool INV, int Ns, class L>
+struct Stages;
+template <int N, bool INV, int Ns, int R, int... Rest>
+struct Stages<N, INV, Ns, RadList<R, Rest...>> {
+    static __device__ __forceinline__ void run(float2 (&v)[FFTCfg<N>::E], float2* lds, int lane,
+                                               const float2* __restrict__ tw) {
+        butterflies<N, INV, Ns, R>(v, lane, tw);
+        if constexpr (sizeof...(Rest) > 0) {
+            exchange<N, Ns, R, RadInfo<RadList<Rest...>>::first>(v, lds, lane);
+            Stages<N, INV, Ns * R, RadList<Rest...>>::run(v, lds, lane, tw);
+        }
+    }
+};
+
+// Transform the vector held by this lane group.  On entry slot e holds element
+// in_index<N>(e, lane); on exit slot e holds output element out_index<N>(e, lane).
+// lds: this group's private region of N float2.
+template <int N, bool INV>
+__device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float2* lds, int lane,
+                                    const float2* __restrict__ tw) {
+    Stages<N, INV, 1, typename FFTCfg<N>::Rad>::run(v, lds, lane, tw);
+}
+
+}  // namespace bimpdev

@@ -1,0 +1,65 @@
+// kernels.hpp -- launch interface between the host context and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace bimpk {
+
+constexpr int kNumOri = 8;
+constexpr int kUniqueOri = 5;  // orientations 5..7 mirror 3..1 in the spectrum
+
+// Per-scale constants of the end-stop / inhibition stencil (host-derived).
+struct CellParams {
+    int ea[kNumOri], eb[kNumOri];  // end-stop offsets (rows, cols)
+    int ia[kNumOri], ib[kNumOri];  // inhibition offsets (rows, cols)
+    int ra[kNumOri], rb[kNumOri];  // radial half offsets
+};
+
+// Per-scale descriptor for the NMS kernels (device table).
+struct NmsScale {
+    const float* D;     // dataH x dataW response map
+    int H, W, off;
+    int block0;         // first NMS block of this scale
+    int nblocks;
+    float lambda, pyrstep;
+    int slot;           // index of the scale's max slot
+};
+
+constexpr int kNmsThreads = 256;
+constexpr int kNmsPerThread = 16;
+constexpr int kNmsBlockPixels = kNmsThreads * kNmsPerThread;
+
+bool fft_size_supported(int n);
+// Register the dynamic-LDS limits of every kernel instance for size n (once per process).
+hipError_t prepare_size(int n);
+
+hipError_t launch_init(unsigned* maxslots, int nslots, int* counter, hipStream_t s);
+hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, uint8_t* dst,
+                           int dw, int dh, size_t dpitch, hipStream_t s);
+hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W, int kX,
+                           float scale, float2* R, int ldr, const float2* tw, hipStream_t s);
+hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, int H, int kY, int ncols,
+                           float2* DT, const float2* tw, hipStream_t s);
+hipError_t launch_inv_cols(int FH, const float2* DT, const float* G, int FW, int Hc, float2* T,
+                           const float2* tw, hipStream_t s);
+hipError_t launch_inv_rows(int FW, const float2* T, int Hc, int Wc, float* C, int ldc,
+                           const float2* tw, hipStream_t s);
+hipError_t launch_cells(const float* C, int Hc, int ldc, int FH, int FW, int H, int W,
+                        const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
+                        hipStream_t s);
+hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s);
+hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_blocks,
+                            const unsigned* maxslots, double thresh, int* block_counts,
+                            hipStream_t s);
+hipError_t launch_nms_scan(int* block_counts, int total_blocks, int* counter, hipStream_t s);
+hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_blocks,
+                            const unsigned* maxslots, double thresh, const int* block_offsets,
+                            void* out, int cap, int matrix_ld, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// column-kernel batch (columns per workgroup) for transform length n
+int col_batch(int n);
+int row_vectors_per_block(int n);
+
+}  // namespace bimpk

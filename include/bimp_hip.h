@@ -1,0 +1,196 @@
+/*
+ * bimp_hip.h -- C ABI of the MI355X-native BIMP keypoint extractor (libbimp_hip.so).
+ *
+ * The drop-in boundary for the reference library daniel-koudouna/cuda-bimp.  Each entry
+ * point names the reference interface it replaces (file:line, basenames: cuda_bimp.hpp =
+ * bimp/include/detection/cuda_bimp.hpp, cuda_bimp.cu = bimp/src/cuda_bimp.cu, bimp.hpp /
+ * bimp.cpp = the Context API, utils.cpp = keypoint layout converters).  Plain pointers and
+ * sizes only; every call returns a status and never exits the process.  The C++ Context
+ * facade over these calls is include/bimp/bimp.hpp.
+ *
+ * Threading: one context per thread/stream.  The library holds no global mutable state;
+ * calls on one context must not overlap.
+ */
+#ifndef BIMP_HIP_H
+#define BIMP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without pulling in the HIP headers. */
+typedef struct ihipStream_t* bimp_stream_t;
+
+#define BIMP_NUM_ORI 8              /* NUM_ORI,          cuda_bimp.hpp:10 */
+#define BIMP_KP_VECTOR_LENGTH 7     /* KP_VECTOR_LENGTH, cuda_bimp.hpp:13 */
+#define BIMP_KP_MAX_POINTS 8000     /* KP_MAX_POINTS,    cuda_bimp.hpp:14 */
+
+/* Status codes (the reference prints and exit()s inside helper_cuda.h macros). */
+enum {
+    BIMP_OK = 0,
+    BIMP_W_OVERFLOW = 1,      /* more keypoints than the cap: count is still the total
+                                 (cuda_bimp.cu:132-138 warns and clamps) */
+    BIMP_E_ARG = -1,          /* invalid argument */
+    BIMP_E_ALLOC = -2,        /* device or host allocation failed */
+    BIMP_E_DEVICE = -3,       /* HIP runtime error (no device, launch failure, ...) */
+    BIMP_E_UNSUPPORTED = -4   /* transform size outside the supported set */
+};
+
+/* Map selectors for bimp_get_map / bimp_set_map. */
+enum {
+    BIMP_MAP_DOUBLE = 0,  /* double-stopped response D after inhibition (d_ResultDouble) */
+    BIMP_MAP_SINGLE = 1,  /* single-stopped response S after inhibition (d_ResultSingle) */
+    BIMP_MAP_COMPLEX = 2  /* complex-cell magnitude C_o of one orientation (d_ResultComplex[o]) */
+};
+
+/* cv::KeyPoint field order: pt.x, pt.y, size, angle, response, octave, class_id
+ * (utils.cpp:130-150 fills cv::KeyPoint from the 7-row SURF_CUDA matrix). */
+typedef struct bimp_keypoint {
+    float x, y;
+    float size;
+    float angle;
+    float response;
+    int octave;
+    int class_id;
+} bimp_keypoint;
+
+/* Run-time options; the reference hard-codes these (cuda_bimp.cu:236-247, :481, :721). */
+typedef struct bimp_options {
+    float gamma;          /* 0.5: envelope aspect (not squared) */
+    float sigma_ratio;    /* 0.56: sigma = sigma_ratio * lambda */
+    float support;        /* 7: filter size = support * lambda (+1 if even) */
+    float pyr_threshold;  /* 7.5: halve the frame while lambda/pyrstep >= this */
+    double nms_threshold; /* 0.104: keypoint threshold relative to the scale's max (the
+                             reference multiplies in double, cuda_bimp.cu:721) */
+    float input_scale;    /* 255/65536: uint8 -> float conversion factor */
+    int flags;            /* BIMP_FLAG_* */
+} bimp_options;
+
+#define BIMP_FLAG_PROFILE 1  /* time every kernel with HIP events (see bimp_get_profile) */
+
+typedef struct bimp_scale_info {
+    float lambda2;   /* requested wavelength (full resolution) */
+    float lambda;    /* wavelength at the pyramid level */
+    float pyrstep;   /* 1, 2, 4, ... */
+    int kernel_size; /* kW (odd) */
+    int kernel_centre;
+    int border;      /* NMS border = (int)lambda */
+    int level;       /* pyramid level = log2(pyrstep) */
+    int data_h, data_w;
+    int fft_h, fft_w;
+    int map_h, map_w, map_pitch; /* stored complex-cell region and its row pitch (floats) */
+} bimp_scale_info;
+
+typedef struct bimp_ctx bimp_ctx;
+
+/* Defaults equal to the reference constants. */
+void bimp_default_options(bimp_options* opt);
+
+/* snapTransformSize (cuda_bimp.hpp:140, cuda_bimp.cu:200-219). */
+int bimp_snap_transform_size(int n);
+
+/* makeLambdasLog (cuda_bimp.hpp:146, cuda_bimp.cu:927-937): writes up to cap wavelengths,
+ * returns how many the reference loop produces. */
+int bimp_make_lambdas_log(float start, float end, int scales_per_octave, float* out, int cap);
+
+/* CudaStuff::CudaStuff(width, height, lambdas) (cuda_bimp.hpp:97, cuda_bimp.cu:222-397):
+ * sorts the wavelengths, derives per-scale parameters, builds the Gabor bank spectra
+ * (FP64 on the host, stored FP32 on the device) and allocates every buffer a frame needs.
+ * opt may be NULL (reference defaults). */
+int bimp_create(int width, int height, const float* lambdas, int n, const bimp_options* opt,
+                bimp_ctx** out);
+
+/* CudaStuff::~CudaStuff (cuda_bimp.cu:399-460). */
+int bimp_destroy(bimp_ctx* ctx);
+
+int bimp_num_scales(const bimp_ctx* ctx);
+int bimp_get_scale_info(const bimp_ctx* ctx, int scale, bimp_scale_info* out);
+
+/* bimp::cuda::getKeypoints (cuda_bimp.hpp:107-111, cuda_bimp.cu:109-140) followed by
+ * utils::downloadKeypoints (utils.cpp:130-150): runs the whole frame (pyramid, Gabor bank,
+ * complex cells, end-stopping, inhibition, NMS) on `stream` and writes min(total, cap)
+ * keypoints ordered by (scale, row, column).  img is width x height uint8 with row pitch
+ * `pitch` bytes, on the device when img_on_device != 0.  out is device memory when
+ * out_on_device != 0 (then *count is written after a stream synchronisation).
+ * Returns BIMP_W_OVERFLOW when total > cap; *count is the total either way. */
+int bimp_detect_u8(bimp_ctx* ctx, const uint8_t* img, size_t pitch, int img_on_device,
+                   bimp_keypoint* out, int cap, int out_on_device, int* count,
+                   bimp_stream_t stream);
+
+/* Same frame with the 7-row SURF_CUDA keypoint matrix the reference returns from
+ * getKeypoints (cuda_bimp.cu:129-140): rows X, Y, LAPLACIAN (int bits), OCTAVE (int bits),
+ * SIZE, ANGLE, HESSIAN, each of leading dimension ld >= cap floats. */
+int bimp_detect_u8_matrix(bimp_ctx* ctx, const uint8_t* img, size_t pitch, int img_on_device,
+                          float* mat, int ld, int cap, int mat_on_device, int* count,
+                          bimp_stream_t stream);
+
+/* Stage entry points of DoFullGPUFiltering (cuda_bimp.cu:462-559), for callers that want
+ * the response maps (Context::getResponses, bimp.cpp:94-104) or that inject maps:
+ *   bimp_filter_u8  -- pyramid + filter bank + complex cells (cuda_bimp.cu:466-524)
+ *   bimp_cells      -- end-stopping + inhibition from the current complex maps (:526-552)
+ *   bimp_keypoints  -- NMS + compaction from the current D maps (:554, :714-743) */
+int bimp_filter_u8(bimp_ctx* ctx, const uint8_t* img, size_t pitch, int img_on_device,
+                   bimp_stream_t stream);
+int bimp_cells(bimp_ctx* ctx, bimp_stream_t stream);
+int bimp_keypoints(bimp_ctx* ctx, bimp_keypoint* out, int cap, int out_on_device, int* count,
+                   bimp_stream_t stream);
+
+/* Read or replace one response map of one scale (d_ResultDouble / d_ResultSingle /
+ * d_ResultComplex[o], cuda_bimp.hpp:50-53).  DOUBLE/SINGLE maps are data_h x data_w;
+ * COMPLEX maps are map_h x map_w.  pitch is in floats.  Synchronous on `stream`. */
+int bimp_get_map(bimp_ctx* ctx, int scale, int which, int orientation, float* dst,
+                 size_t pitch, int dst_on_device, bimp_stream_t stream);
+int bimp_set_map(bimp_ctx* ctx, int scale, int which, int orientation, const float* src,
+                 size_t pitch, int src_on_device, bimp_stream_t stream);
+
+/* utils::uploadKeypoints / downloadKeypoints layout converters (utils.cpp:109-150),
+ * host memory: keypoints <-> 7 x ld SURF_CUDA matrix. */
+int bimp_keypoints_to_matrix(const bimp_keypoint* kps, int n, float* mat, int ld);
+int bimp_matrix_to_keypoints(const float* mat, int ld, int n, bimp_keypoint* kps);
+
+const char* bimp_last_error(const bimp_ctx* ctx);
+const char* bimp_status_string(int status);
+
+/* ---- profiling (replaces utils::log / showTimes, utils.cpp:11-30, :54-85) ---- */
+typedef struct bimp_profile_entry {
+    char name[32];      /* kernel family */
+    int launches;       /* launches accumulated */
+    double total_ms;    /* summed HIP-event time */
+    double bytes;       /* summed algorithmic bytes of those launches */
+} bimp_profile_entry;
+/* With BIMP_FLAG_PROFILE set, every launch is bracketed by HIP events on the caller's stream;
+ * this returns the accumulated per-family totals (synchronises the last stream used). */
+int bimp_get_profile(bimp_ctx* ctx, bimp_profile_entry* out, int cap, int* n);
+int bimp_reset_profile(bimp_ctx* ctx);
+int bimp_set_flags(bimp_ctx* ctx, int flags);
+
+/* ---- device plumbing for hosts without a GPU array library (no counterpart in the
+ * reference, which relies on cv::cuda::GpuMat) ---- */
+int bimp_device_count(int* n);
+int bimp_device_malloc(void** p, size_t bytes);
+int bimp_device_free(void* p);
+int bimp_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int bimp_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int bimp_stream_create(bimp_stream_t* s);
+int bimp_stream_destroy(bimp_stream_t s);
+int bimp_stream_sync(bimp_stream_t s);
+/* Time `iters` back-to-back detects (device-resident frame and keypoint buffer) with HIP
+ * events on `stream`; writes the elapsed milliseconds. */
+int bimp_time_detect(bimp_ctx* ctx, const uint8_t* d_img, size_t pitch, bimp_keypoint* d_out,
+                     int cap, int iters, bimp_stream_t stream, float* elapsed_ms);
+/* A plain device copy kernel of known size, used to calibrate HBM byte counters. */
+int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t stream);
+
+/* Seeded synthetic frame generator for tests and benchmarks (no counterpart in the
+ * reference, which ships no images): background U[0,255], nshapes rectangles/ellipses,
+ * N(0,5^2) noise, uint8.  Host memory. */
+int bimp_synth_frame(int width, int height, int nshapes, uint64_t seed, uint8_t* out,
+                     size_t pitch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

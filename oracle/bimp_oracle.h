@@ -1,0 +1,95 @@
+/*
+ * bimp_oracle.h -- CPU restatement of the reference BIMP keypoint path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke check and the
+ * CPU-baseline leg of bench.py load it to compare against the HIP library.  The product
+ * (cuda-bimp_amd/) never links, loads or calls it.
+ *
+ * Every function restates the reference semantics it cites (daniel-koudouna/cuda-bimp;
+ * file:line with basenames: cuda_bimp.cu = bimp/src/cuda_bimp.cu, cuda_bimp.cuh =
+ * bimp/include/detection/cuda_bimp.cuh, convolutionFFT2D.cuh =
+ * bimp/include/detection/convolutionFFT2D.cuh).  The reference's FFT (cuFFT), pyramid
+ * (OpenCV-CUDA pyrDown) and max reduction (thrust) are third-party and cannot run here;
+ * the oracle restates their published mathematics instead:
+ *   - FFT convolution: textbook DFT, own mixed-radix Stockham FFT (FP64 or FP32);
+ *   - pyrDown: 5-tap binomial [1 4 6 4 1]^2/256, reflect-101 border, round-half-even
+ *     (OpenCV-CUDA 3.3.1 semantics; parity unpinned: no OpenCV in this image);
+ *   - max: exact float maximum.
+ */
+#ifndef BIMP_ORACLE_H
+#define BIMP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cv::KeyPoint field order (pt.x, pt.y, size, angle, response, octave, class_id). */
+typedef struct {
+    float x, y, size, angle, response;
+    int octave, class_id;
+} bo_keypoint;
+
+/* Per-scale parameters derived exactly as in CudaStuff::CudaStuff (cuda_bimp.cu:222-256). */
+typedef struct {
+    float lambda2;  /* wavelength requested at full resolution */
+    float lambda;   /* wavelength at the pyramid level */
+    float pyrstep;  /* 1, 2, 4, ... */
+    float fs;       /* float filter size (7*lambda, +1 if its integer part is even) */
+    int kW;         /* kernel width = height (odd) */
+    int kX;         /* kernel centre = floor(fs/2) */
+    int off;        /* NMS border = (int)lambda */
+    int level;      /* log2(pyrstep) */
+    int dataH, dataW;
+    int fftH, fftW;
+} bo_scale;
+
+int bo_snap_transform_size(int n);
+int bo_make_lambdas_log(float start, float end, int spo, float* out, int cap);
+int bo_scale_params(float lambda2, int width, int height, bo_scale* s);
+void bo_gabor(const bo_scale* s, int o, float* ke, float* ko);
+int bo_pyr_down(const uint8_t* src, int w, int h, size_t pitch, uint8_t* dst);
+void bo_pad_clamp(const uint8_t* img, size_t pitch, const bo_scale* s, float* out);
+int bo_end_stop_offsets(float lambda, int o, int* a, int* b);
+int bo_inhibition_offsets(float lambda, int o, int* a, int* b);
+
+/* In-place 1D complex FFT in FP64 on interleaved (re, im) pairs. sign -1 forward, +1 inverse
+ * (unnormalised).  Returns 0, or -1 for an unsupported length. */
+int bo_fft_c2c(double* data, int n, int sign);
+
+/* Brute-force circular convolution over the FH x FW torus of the padded image with the
+ * padded kernel (the mathematics the FFT path must equal; O(FH*FW*kW^2), tiny inputs only).
+ * e and o are FP64 maps of FH*FW. */
+void bo_brute_filter(const uint8_t* level_img, size_t pitch, const bo_scale* s, int o,
+                     double* e, double* od);
+
+/* ---- context: bank precomputed at creation, as CudaStuff does ---- */
+typedef struct bo_ctx bo_ctx;
+bo_ctx* bo_create(int width, int height, const float* lambdas, int n, int fp64);
+void bo_destroy(bo_ctx* c);
+int bo_num_scales(const bo_ctx* c);
+int bo_get_scale(const bo_ctx* c, int i, bo_scale* out);
+int bo_set_threads(int n);
+
+/* Stage 0: pyramid levels and per-scale crops of the uint8 frame. */
+int bo_set_image(bo_ctx* c, const uint8_t* img, size_t pitch);
+/* Copy the uint8 data image of scale i (dataH x dataW) after bo_set_image. */
+int bo_get_level_image(const bo_ctx* c, int i, uint8_t* dst);
+/* Stage 1: complex-cell maps of scale i, 8 maps of fftH x fftW (row stride fftW). */
+int bo_filter_scale(bo_ctx* c, int i, float* maps);
+/* Stage 2: end-stopping, inhibition and inhibit from 8 fftH x fftW maps -> D, S (dataH x dataW). */
+int bo_cells_scale(const bo_ctx* c, int i, const float* maps, float* D, float* S);
+/* Stage 3: NMS of D for scale i; appends at out[*count..] (writes only below cap), *count
+ * grows by the number of keypoints found. */
+int bo_nms_scale(const bo_ctx* c, int i, const float* D, bo_keypoint* out, int cap, int* count);
+/* Whole frame: stages 0-3 over every scale in ascending lambda order.  D_out / S_out, if
+ * non-null, receive the per-scale maps concatenated (sum of dataH*dataW). */
+int bo_detect(bo_ctx* c, const uint8_t* img, size_t pitch, bo_keypoint* out, int cap,
+              int* count, float* D_out, float* S_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
