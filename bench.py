@@ -1,0 +1,170 @@
+"""BIMP keypoint extraction benchmark on one MI355X.
+
+Workload: one 1920x1080 uint8 frame (300 synthetic shapes, seed 1), bank of 8 wavelengths
+lambda = 4*2^(k/4) x 8 orientations, plain FFT-FP32 path.  A step is one full frame: pyramid,
+filter bank, complex cells, end-stopping, inhibition, NMS and keypoint compaction, from a
+device-resident frame to a device-resident keypoint list and count.
+
+Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel, CPU baseline).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-bimp_amd"))
+
+METRIC = "BIMP frames/s at 1920x1080 (FFT-FP32 vs BF16-MFMA) + % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+W, H, NSHAPES, SEED = 1920, 1080, 300, 1
+
+
+def bank():
+    import numpy as np
+    return np.array([4.0 * 2.0 ** (k / 4.0) for k in range(8)], np.float64).astype(np.float32)
+
+
+def b_alg(cs):
+    """Stage-boundary algorithmic bytes of one frame: W*H + sum_s (84*A + 352*C)."""
+    tot = W * H
+    for si in cs.ks:
+        A = si.data_h * si.data_w
+        C = si.fft_h * (si.fft_w // 2 + 1)
+        tot += 84 * A + 352 * C
+    return tot
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(img, frames):
+    """The CPU restatement (FP32 FFT path, OpenMP over rows/columns/pixels) on the host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = min(16, os.cpu_count() or 1)
+    O.lib().bo_set_threads(threads)
+    orc = O.Oracle(W, H, bank(), fp64=False)
+    orc.detect(img)  # first frame warms caches and page mappings
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        _, n = orc.detect(img)
+    dt = time.perf_counter() - t0
+    return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames of 1920x1080 after one warm-up frame, FP32 FFT "
+                      f"restatement, {threads} OpenMP threads on {cpu_model()} "
+                      f"(nproc {os.cpu_count()}), {n} keypoints/frame"}
+
+
+def load_traffic(kernel):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-frames", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(frames and scales are independent: run one process per GPU).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import bimp_amd as B
+
+    if B.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    img = B.synth_frame(W, H, NSHAPES, SEED)
+    cs = B.CudaStuff(W, H, bank())
+    stream = B.Stream()
+    d_img = B.DeviceBuffer(W * H)
+    d_img.upload(img)
+    cap = B.KP_MAX_POINTS
+    d_out = B.DeviceBuffer(cap * B.KEYPOINT_DTYPE.itemsize)
+
+    # warm-up, then the timed region (HIP events on the launch stream, synchronised on both sides)
+    if args.warmup:
+        cs.time_detect(d_img, W, d_out, cap, args.warmup, stream)
+    stream.sync()
+    t0 = time.perf_counter()
+    ms = cs.time_detect(d_img, W, d_out, cap, args.steps, stream)
+    stream.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / args.steps
+    fps = 1000.0 / ms_per_step
+    kps, count = cs.detect(img)
+
+    # per-kernel profile (HIP events around every launch), separate from the timed region
+    cs.set_flags(B.FLAG_PROFILE)
+    cs.reset_profile()
+    cs.time_detect(d_img, W, d_out, cap, args.profile_frames, stream)
+    prof = cs.profile()
+    cs.set_flags(0)
+    kern = {k: v for k, v in prof.items() if k.startswith("k_")}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    d = kern[dom]
+    avg_ms = d["total_ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    frame_kernel_ms = sum(v["total_ms"] for v in kern.values()) / args.profile_frames
+    balg = b_alg(cs)
+
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "BIMP plain FFT-FP32, one 1920x1080 uint8 frame, 300 shapes, "
+                               "seed 1, 8 wavelengths 4*2^(k/4) x 8 orientations",
+                   "frame": [W, H], "scales": len(cs.ks), "orientations": 8,
+                   "keypoints": int(count)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(dom),
+                     "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "alg_bytes_per_launch": int(bytes_per_launch)},
+        "frame_b_alg": {"bytes": int(balg), "gbs": round(balg / (ms_per_step * 1e-3) / 1e9, 1),
+                        "frac": round(balg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels_ms_per_frame": {k: round(v["total_ms"] / args.profile_frames, 4)
+                                 for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])},
+        "kernel_sum_ms_per_frame": round(frame_kernel_ms, 4),
+        "wall_s_timed": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(img, args.cpu_frames)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,9 +28,10 @@ enum Family {
     F_INIT, F_COPY_IN, F_PYR, F_FWD_ROWS, F_FWD_COLS, F_INV_COLS, F_INV_ROWS, F_CELLS,
     F_MAX, F_NMS_COUNT, F_NMS_SCAN, F_NMS_WRITE, F_COUNT
 };
-const char* kFamilyNames[F_COUNT] = {"init",     "copy_in",  "pyr_down",  "fwd_rows",
-                                     "fwd_cols", "inv_cols", "inv_rows",  "cells",
-                                     "max",      "nms_count", "nms_scan", "nms_write"};
+// Profile names follow the kernel symbols (size-templated kernels carry their length).
+const char* kFamilyNames[F_COUNT] = {"k_init",      "copy_in",    "k_pyr_down",  "k_fwd_rows",
+                                     "k_fwd_cols",  "k_inv_cols", "k_inv_rows",  "k_cells",
+                                     "k_max",       "k_nms_count", "k_nms_scan", "k_nms_write"};
 
 struct Scale {
     bimph::ScaleGeom g;
@@ -45,8 +46,13 @@ struct Scale {
 
 struct ProfEv {
     hipEvent_t a, b;
-    int fam;
+    int fam, size;
     double bytes;
+};
+
+struct ProfAcc {
+    int launches = 0;
+    double ms = 0, bytes = 0;
 };
 
 }  // namespace
@@ -73,9 +79,7 @@ struct bimp_ctx {
     // profiling
     std::vector<ProfEv> ev;
     size_t evUsed = 0;
-    double profMs[F_COUNT] = {0};
-    double profBytes[F_COUNT] = {0};
-    int profN[F_COUNT] = {0};
+    std::map<std::string, ProfAcc> prof;
     hipStream_t lastStream = nullptr;
 };
 
@@ -108,7 +112,7 @@ struct LaunchScope {
     bimp_ctx* c;
     hipStream_t s;
     ProfEv* e = nullptr;
-    LaunchScope(bimp_ctx* c_, hipStream_t s_, int fam, double bytes) : c(c_), s(s_) {
+    LaunchScope(bimp_ctx* c_, hipStream_t s_, int fam, double bytes, int size = 0) : c(c_), s(s_) {
         if (!(c->opt.flags & BIMP_FLAG_PROFILE)) return;
         if (c->evUsed == c->ev.size()) {
             ProfEv n;
@@ -118,6 +122,7 @@ struct LaunchScope {
         }
         e = &c->ev[c->evUsed++];
         e->fam = fam;
+        e->size = size;
         e->bytes = bytes;
         (void)hipEventRecord(e->a, s);
     }
@@ -132,9 +137,12 @@ void collect_profile(bimp_ctx* c) {
     for (size_t i = 0; i < c->evUsed; ++i) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, c->ev[i].a, c->ev[i].b) == hipSuccess) {
-            c->profMs[c->ev[i].fam] += ms;
-            c->profBytes[c->ev[i].fam] += c->ev[i].bytes;
-            c->profN[c->ev[i].fam] += 1;
+            std::string name = kFamilyNames[c->ev[i].fam];
+            if (c->ev[i].size) name += "<" + std::to_string(c->ev[i].size) + ">";
+            ProfAcc& a = c->prof[name];
+            a.ms += ms;
+            a.bytes += c->ev[i].bytes;
+            a.launches += 1;
         }
     }
     c->evUsed = 0;
@@ -197,23 +205,24 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hip
         const float2* twW = c->tw[g.FW];
         const float2* twH = c->tw[g.FH];
         {
-            LaunchScope ls(c, s, F_FWD_ROWS, (double)g.H * g.W + 8.0 * g.H * ncols);
+            LaunchScope ls(c, s, F_FWD_ROWS, (double)g.H * g.W + 8.0 * g.H * ncols, g.FW);
             HIPCHK(c, bimpk::launch_fwd_rows(g.FW, lv[g.level], lp[g.level], g.H, g.W, g.kX,
                                              c->opt.input_scale, c->R, ncols, twW, s));
         }
         {
-            LaunchScope ls(c, s, F_FWD_COLS, 8.0 * g.H * ncols + 8.0 * ncols * g.FH);
+            LaunchScope ls(c, s, F_FWD_COLS, 8.0 * g.H * ncols + 8.0 * ncols * g.FH, g.FH);
             HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, g.H, g.kX, ncols, sc.DT, twH, s));
         }
         {
             LaunchScope ls(c, s, F_INV_COLS,
                            8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
-                               8.0 * kNumOri * sc.Hc * g.FW);
+                               8.0 * kNumOri * sc.Hc * g.FW,
+                           g.FH);
             HIPCHK(c, bimpk::launch_inv_cols(g.FH, sc.DT, sc.G, g.FW, sc.Hc, c->T, twH, s));
         }
         {
             LaunchScope ls(c, s, F_INV_ROWS,
-                           8.0 * kNumOri * sc.Hc * g.FW + 4.0 * kNumOri * sc.Hc * sc.Wc);
+                           8.0 * kNumOri * sc.Hc * g.FW + 4.0 * kNumOri * sc.Hc * sc.Wc, g.FW);
             HIPCHK(c, bimpk::launch_inv_rows(g.FW, c->T, sc.Hc, sc.Wc, sc.C, sc.ldc, twW, s));
         }
     }
@@ -661,13 +670,12 @@ int bimp_get_profile(bimp_ctx* c, bimp_profile_entry* out, int cap, int* n) {
     if (!c || !n) return BIMP_E_ARG;
     collect_profile(c);
     int k = 0;
-    for (int f = 0; f < F_COUNT; ++f) {
-        if (!c->profN[f]) continue;
+    for (const auto& kv : c->prof) {
         if (out && k < cap) {
-            std::snprintf(out[k].name, sizeof(out[k].name), "%s", kFamilyNames[f]);
-            out[k].launches = c->profN[f];
-            out[k].total_ms = c->profMs[f];
-            out[k].bytes = c->profBytes[f];
+            std::snprintf(out[k].name, sizeof(out[k].name), "%s", kv.first.c_str());
+            out[k].launches = kv.second.launches;
+            out[k].total_ms = kv.second.ms;
+            out[k].bytes = kv.second.bytes;
         }
         ++k;
     }
@@ -678,11 +686,7 @@ int bimp_get_profile(bimp_ctx* c, bimp_profile_entry* out, int cap, int* n) {
 int bimp_reset_profile(bimp_ctx* c) {
     if (!c) return BIMP_E_ARG;
     collect_profile(c);
-    for (int f = 0; f < F_COUNT; ++f) {
-        c->profMs[f] = 0;
-        c->profBytes[f] = 0;
-        c->profN[f] = 0;
-    }
+    c->prof.clear();
     return BIMP_OK;
 }
 
