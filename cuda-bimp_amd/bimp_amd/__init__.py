@@ -128,7 +128,7 @@ def _ptr(a):
 
 def _check(st, ctx=None):
     if st < 0:
-        msg = lib.bimp_last_error(ctx).decode() if ctx else lib.bimp_status_string(st).decode()
+        msg = lib.bimp_last_error(ctx).decode()
         raise BimpError(st, msg or lib.bimp_status_string(st).decode())
     return st
 

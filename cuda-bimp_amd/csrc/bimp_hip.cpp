@@ -44,6 +44,11 @@ struct Scale {
     float* S = nullptr;    // H x W
 };
 
+// Scales sharing pyramid level and transform size; each FFT stage runs once per group.
+struct Group {
+    int first, n;
+};
+
 struct ProfEv {
     hipEvent_t a, b;
     int fam, size;
@@ -61,6 +66,7 @@ struct bimp_ctx {
     int width = 0, height = 0;
     bimp_options opt;
     std::vector<Scale> sc;
+    std::vector<Group> groups;
     int maxLevel = 0;
     std::vector<uint8_t*> level;  // level[0] is the staging copy of a host frame
     std::vector<int> levelW, levelH;
@@ -84,6 +90,9 @@ struct bimp_ctx {
 };
 
 namespace {
+
+// Message of the last failed bimp_create on this thread (the context does not exist then).
+thread_local std::string g_create_error;
 
 int fail(bimp_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -199,31 +208,44 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hip
         lv[l] = c->level[l];
         lp[l] = (size_t)c->levelW[l];
     }
-    for (auto& sc : c->sc) {
-        const auto& g = sc.g;
+    for (const auto& gr : c->groups) {
+        const auto& g = c->sc[gr.first].g;
+        const Scale& s0 = c->sc[gr.first];
         const int ncols = g.FW / 2 + 1;
         const float2* twW = c->tw[g.FW];
         const float2* twH = c->tw[g.FH];
+        bimpk::ScaleSet ss;
+        ss.n = gr.n;
+        for (int k = 0; k < gr.n; ++k) {
+            const Scale& sk = c->sc[gr.first + k];
+            ss.kX[k] = sk.g.kX;
+            ss.G[k] = sk.G;
+            ss.DT[k] = sk.DT;
+            ss.C[k] = sk.C;
+        }
+        const size_t rstride = (size_t)g.H * ncols;
+        const size_t tstride = (size_t)kNumOri * s0.Hc * g.FW;
+        const double n = gr.n;
         {
-            LaunchScope ls(c, s, F_FWD_ROWS, (double)g.H * g.W + 8.0 * g.H * ncols, g.FW);
-            HIPCHK(c, bimpk::launch_fwd_rows(g.FW, lv[g.level], lp[g.level], g.H, g.W, g.kX,
-                                             c->opt.input_scale, c->R, ncols, twW, s));
+            LaunchScope ls(c, s, F_FWD_ROWS, n * ((double)g.H * g.W + 8.0 * g.H * ncols), g.FW);
+            HIPCHK(c, bimpk::launch_fwd_rows(g.FW, lv[g.level], lp[g.level], g.H, g.W, ss,
+                                             c->opt.input_scale, c->R, ncols, rstride, twW, s));
         }
         {
-            LaunchScope ls(c, s, F_FWD_COLS, 8.0 * g.H * ncols + 8.0 * ncols * g.FH, g.FH);
-            HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, g.H, g.kX, ncols, sc.DT, twH, s));
+            LaunchScope ls(c, s, F_FWD_COLS, n * (8.0 * g.H * ncols + 8.0 * ncols * g.FH), g.FH);
+            HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, rstride, g.H, ss, ncols, twH, s));
         }
         {
             LaunchScope ls(c, s, F_INV_COLS,
-                           8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
-                               8.0 * kNumOri * sc.Hc * g.FW,
+                           n * (8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
+                                8.0 * kNumOri * s0.Hc * g.FW),
                            g.FH);
-            HIPCHK(c, bimpk::launch_inv_cols(g.FH, sc.DT, sc.G, g.FW, sc.Hc, c->T, twH, s));
+            HIPCHK(c, bimpk::launch_inv_cols(g.FH, ss, g.FW, s0.Hc, c->T, tstride, twH, s));
         }
         {
             LaunchScope ls(c, s, F_INV_ROWS,
-                           8.0 * kNumOri * sc.Hc * g.FW + 4.0 * kNumOri * sc.Hc * sc.Wc, g.FW);
-            HIPCHK(c, bimpk::launch_inv_rows(g.FW, c->T, sc.Hc, sc.Wc, sc.C, sc.ldc, twW, s));
+                           n * (8.0 * kNumOri * s0.Hc * g.FW + 4.0 * kNumOri * s0.Hc * s0.Wc), g.FW);
+            HIPCHK(c, bimpk::launch_inv_rows(g.FW, ss, c->T, tstride, s0.Hc, s0.Wc, s0.ldc, twW, s));
         }
     }
     return BIMP_OK;
@@ -354,7 +376,10 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
                 bimp_ctx** out) {
     if (!out) return BIMP_E_ARG;
     *out = nullptr;
-    if (width < 4 || height < 4 || !lambdas || n <= 0 || n > 64) return BIMP_E_ARG;
+    if (width < 4 || height < 4 || !lambdas || n <= 0 || n > 64) {
+        g_create_error = "invalid frame size or wavelength list";
+        return BIMP_E_ARG;
+    }
     bimp_ctx* c = new bimp_ctx;
     c->width = width;
     c->height = height;
@@ -369,11 +394,14 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         if (!bimph::derive_scale(l, width, height, c->opt.pyr_threshold, c->opt.support, &s.g) ||
             s.g.lambda < 1.0f || s.g.H < 4 || s.g.W < 4 || s.g.kW < 1) {
             delete c;
+            g_create_error = "wavelength " + std::to_string(l) + " gives an invalid scale";
             return BIMP_E_ARG;
         }
         if (!bimpk::fft_size_supported(s.g.FH) || !bimpk::fft_size_supported(s.g.FW) ||
             s.g.FW % bimpk::col_batch(s.g.FH) != 0) {
             delete c;
+            g_create_error = "transform size " + std::to_string(s.g.FH) + " x " +
+                             std::to_string(s.g.FW) + " is not supported";
             return BIMP_E_UNSUPPORTED;
         }
         const auto& g = s.g;
@@ -395,11 +423,11 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     auto bail = [&](int code, const std::string& msg) {
         free_all(c);
         delete c;
-        (void)msg;
+        g_create_error = msg;
         return code;
     };
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bail(BIMP_E_DEVICE, "no device");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bail(BIMP_E_DEVICE, "no HIP device");
     // pyramid levels
     for (auto& s : c->sc) c->maxLevel = std::max(c->maxLevel, s.g.level);
     c->levelW.resize(c->maxLevel + 1);
@@ -414,15 +442,32 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     for (int l = 0; l <= c->maxLevel; ++l)
         if (dalloc(&c->level[l], (size_t)c->levelW[l] * c->levelH[l]) != hipSuccess)
             return bail(BIMP_E_ALLOC, "level");
+    // resolution groups: consecutive scales (ascending lambda) with one level and size
+    for (int i = 0; i < (int)c->sc.size(); ++i) {
+        const auto& g = c->sc[i].g;
+        if (!c->groups.empty()) {
+            Group& last = c->groups.back();
+            const auto& h0 = c->sc[last.first].g;
+            if (h0.level == g.level && h0.H == g.H && h0.W == g.W && h0.FH == g.FH &&
+                h0.FW == g.FW && last.n < bimpk::kMaxBatch) {
+                ++last.n;
+                continue;
+            }
+        }
+        c->groups.push_back({i, 1});
+    }
     size_t rMax = 0, tMax = 0;
     std::vector<int> sizes;
     const unsigned hw = std::thread::hardware_concurrency();
     const int threads = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+    for (const auto& gr : c->groups) {
+        const Scale& s0 = c->sc[gr.first];
+        rMax = std::max(rMax, (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
+        tMax = std::max(tMax, (size_t)gr.n * kNumOri * s0.Hc * s0.g.FW);
+    }
     for (auto& s : c->sc) {
         const auto& g = s.g;
         const int ncols = g.FW / 2 + 1;
-        rMax = std::max(rMax, (size_t)g.H * ncols);
-        tMax = std::max(tMax, (size_t)kNumOri * s.Hc * g.FW);
         sizes.push_back(g.FH);
         sizes.push_back(g.FW);
         if (dalloc(&s.G, (size_t)kUniqueOri * g.FW * g.FH) != hipSuccess ||
@@ -442,19 +487,19 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     }
     if (dalloc(&c->R, rMax) != hipSuccess || dalloc(&c->T, tMax) != hipSuccess)
         return bail(BIMP_E_ALLOC, "scratch");
+    if (hipError_t e = bimpk::prepare_common(); e != hipSuccess)
+        return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));
     std::sort(sizes.begin(), sizes.end());
     sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
     for (int n2 : sizes) {
-        if (bimpk::prepare_size(n2) != hipSuccess) return bail(BIMP_E_DEVICE, "kernel attributes");
-        std::vector<float2> t(n2);
-        for (int k = 0; k < n2; ++k) {
-            const double a = -2.0 * 3.1415926535897932384626433832795 * k / n2;
-            t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
+        if (hipError_t e = bimpk::prepare_size(n2); e != hipSuccess)
+            return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));
+        std::vector<float2> t;
+        if (!bimpk::twiddle_table(n2, t)) return bail(BIMP_E_UNSUPPORTED, "twiddle table");
         float2* d = nullptr;
-        if (dalloc(&d, n2) != hipSuccess) return bail(BIMP_E_ALLOC, "twiddles");
+        if (dalloc(&d, t.size()) != hipSuccess) return bail(BIMP_E_ALLOC, "twiddles");
         c->tw[n2] = d;
-        if (hipMemcpy(d, t.data(), n2 * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(d, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
             return bail(BIMP_E_DEVICE, "twiddle upload");
     }
     // NMS descriptors
@@ -652,7 +697,7 @@ int bimp_matrix_to_keypoints(const float* mat, int ld, int n, bimp_keypoint* k) 
     return BIMP_OK;
 }
 
-const char* bimp_last_error(const bimp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* bimp_last_error(const bimp_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
 const char* bimp_status_string(int s) {
     switch (s) {

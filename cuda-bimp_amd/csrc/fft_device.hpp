@@ -3,18 +3,19 @@
 // A length-N transform is owned by a group of TPV lanes (TPV = 64 for N >= 512: one
 // wavefront per vector); every lane keeps E = N/TPV complex values in registers.  Each
 // radix-R stage does E/R butterflies per lane entirely in registers; between stages the
-// group exchanges data through its own LDS region (N float2), so the only cross-lane
-// traffic is one LDS write + read per stage boundary.
+// group exchanges data through its own LDS region (N floats: real parts, then imaginary
+// parts), so the only cross-lane traffic is one LDS write + read per stage boundary.
 //
 // Index maps (Stockham autosort, radix list R0..RL, Ns = product of earlier radices):
 //   stage input  of butterfly j = lane + TPV*m, leg r : j + r*N/R
 //   stage output of butterfly j, leg r              : (j/Ns)*Ns*R + j%Ns + r*Ns
 // so the first stage reads element in_index(e) and the last stage produces element
 // out_index(e) in register slot e -- both are unit-stride across lanes, which makes global
-// loads and stores of a vector coalesced without any extra shuffle.
+// loads and stores of a vector coalesced without any extra shuffle.  A group never spans
+// two wavefronts, so stage boundaries need only wavefront-level synchronisation.
 //
-// Twiddles come from a per-N table tw[k] = exp(-2*pi*i*k/N) computed in FP64 on the host
-// and rounded to FP32; the inverse transform uses their conjugates (unnormalised).
+// Twiddles come from a per-N packed table (TwLayout) computed in FP64 on the host and
+// rounded to FP32; the inverse transform uses their conjugates (unnormalised).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -188,6 +189,23 @@ struct RadInfo<RadList<R0>> {
     static constexpr int last = R0;
 };
 
+// Packed per-stage twiddle table.  Stage t (radix R, Ns = product of the earlier radices)
+// keeps, for every k in [0, Ns), the R-1 twiddles exp(-2*pi*i*r*k/(Ns*R)), r = 1..R-1, in a
+// row padded to an even length, so a butterfly fetches its twiddles as float4 pairs from one
+// base address.  Stage 0 (Ns = 1) needs none.  Stages are concatenated in plan order.
+__host__ __device__ constexpr int tw_row(int R) { return (R - 1 + 1) / 2 * 2; }
+
+template <class L, int Ns = 1, bool First = true>
+struct TwLayout;
+template <int R, int... Rest, int Ns, bool First>
+struct TwLayout<RadList<R, Rest...>, Ns, First> {
+    static constexpr int size = (First ? 0 : Ns * tw_row(R)) + TwLayout<RadList<Rest...>, Ns * R, false>::size;
+};
+template <int Ns, bool First>
+struct TwLayout<RadList<>, Ns, First> {
+    static constexpr int size = 0;
+};
+
 // element index held in register slot e before the first stage / after the last stage
 template <int N>
 __device__ __forceinline__ int in_index(int e, int lane) {
@@ -202,22 +220,33 @@ __device__ __forceinline__ int out_index(int e, int lane) {
     return lane + C::TPV * (e / R) + (e % R) * (N / R);
 }
 
-// LDS ordering between the stages of one group.  Groups never share LDS regions, but a
-// workgroup barrier keeps the code independent of wave-level ordering assumptions.
-__device__ __forceinline__ void stage_sync() { __syncthreads(); }
+// LDS ordering between the stages of one group.  A group never spans two wavefronts (TPV
+// divides 64) and owns its LDS region, so a wavefront-scope release/acquire around a wave
+// barrier orders the exchange (the same sequence HIP uses for __syncwarp).
+__device__ __forceinline__ void stage_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int N, bool INV, int Ns, int R>
 __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
                                             const float2* __restrict__ tw) {
     using C = FFTCfg<N>;
+    constexpr int P = tw_row(R);
 #pragma unroll
     for (int m = 0; m < C::E / R; ++m) {
         if constexpr (Ns > 1) {
             const int j = lane + C::TPV * m;
             const int k = j % Ns;
+            const float4* row = reinterpret_cast<const float4*>(tw + k * P);
+            float4 t[P / 2];
+#pragma unroll
+            for (int q = 0; q < P / 2; ++q) t[q] = row[q];
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * (N / (Ns * R))];
+                const float4 tq = t[(r - 1) / 2];
+                float2 w = ((r - 1) & 1) ? make_float2(tq.z, tq.w) : make_float2(tq.x, tq.y);
                 if (INV) w.y = -w.y;
                 v[m * R + r] = cmul(v[m * R + r], w);
             }
@@ -226,22 +255,43 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
     }
 }
 
+// The exchange moves real parts, then imaginary parts, through an LDS region of N floats:
+// a register slot keeps its old imaginary part until the second half has been written, so
+// half the LDS of a complex exchange suffices.
 template <int N, int Ns, int R, int R2>
-__device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float2* lds, int lane) {
+__device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, int lane) {
     using C = FFTCfg<N>;
 #pragma unroll
     for (int m = 0; m < C::E / R; ++m) {
         const int j = lane + C::TPV * m;
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[base + r * Ns] = v[m * R + r];
+        for (int r = 0; r < R; ++r) lds[base + r * Ns] = v[m * R + r].x;
+    }
+    stage_sync();
+    float im[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) im[e] = v[e].y;
+#pragma unroll
+    for (int m = 0; m < C::E / R2; ++m) {
+        const int j = lane + C::TPV * m;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[j + r * (N / R2)];
+    }
+    stage_sync();
+#pragma unroll
+    for (int m = 0; m < C::E / R; ++m) {
+        const int j = lane + C::TPV * m;
+        const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[base + r * Ns] = im[m * R + r];
     }
     stage_sync();
 #pragma unroll
     for (int m = 0; m < C::E / R2; ++m) {
         const int j = lane + C::TPV * m;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r] = lds[j + r * (N / R2)];
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[j + r * (N / R2)];
     }
     stage_sync();
 }
@@ -250,22 +300,26 @@ template <int N, bool INV, int Ns, class L>
 struct Stages;
 template <int N, bool INV, int Ns, int R, int... Rest>
 struct Stages<N, INV, Ns, RadList<R, Rest...>> {
-    static __device__ __forceinline__ void run(float2 (&v)[FFTCfg<N>::E], float2* lds, int lane,
+    // tw points at this stage's rows (unused for the first stage)
+    static __device__ __forceinline__ void run(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
                                                const float2* __restrict__ tw) {
         butterflies<N, INV, Ns, R>(v, lane, tw);
         if constexpr (sizeof...(Rest) > 0) {
             exchange<N, Ns, R, RadInfo<RadList<Rest...>>::first>(v, lds, lane);
-            Stages<N, INV, Ns * R, RadList<Rest...>>::run(v, lds, lane, tw);
+            constexpr int next_ns = Ns * R;
+            constexpr int adv = Ns > 1 ? Ns * tw_row(R) : 0;
+            Stages<N, INV, next_ns, RadList<Rest...>>::run(v, lds, lane, tw + adv);
         }
     }
 };
 
-// Transform the vector held by this lane group.  On entry slot e holds element
-// in_index<N>(e, lane); on exit slot e holds output element out_index<N>(e, lane).
-// lds: this group's private region of N float2.
+// The twiddle pointer is passed through an empty asm statement so that a kernel running
+// several transforms in a loop reloads twiddles (L1 hits) instead of keeping every one of
+// them live in registers across iterations.
 template <int N, bool INV>
-__device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float2* lds, int lane,
-                                    const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
+                                    const float2* tw) {
+    asm volatile("" : "+s"(tw));
     Stages<N, INV, 1, typename FFTCfg<N>::Rad>::run(v, lds, lane, tw);
 }
 

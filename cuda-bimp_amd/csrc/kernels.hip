@@ -10,6 +10,10 @@
 //   k_inv_rows     inverse row FFTs, |e + i*o| -> complex-cell map C_o (complexResponse, :518)
 //   k_cells        end-stopping + tangential/radial inhibition + inhibit + max (:526-552, :720)
 //   k_nms_*        threshold, strict 8-neighbour max, parabola, ordered compaction (:714-743)
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "fft_device.hpp"
 #include "kernels.hpp"
 
@@ -50,21 +54,31 @@ __global__ void k_pyr_down(const uint8_t* __restrict__ src, int sw, int sh, size
 }
 
 // ------------------------------------------------------------------ forward rows
+template <int N>
+struct RowCfg {
+    static constexpr int TPV = FFTCfg<N>::TPV;
+    static constexpr int THREADS = 64;  // one wavefront per workgroup
+    static constexpr int VPB = THREADS / TPV;
+};
+
 // Two image rows (a, b) form one complex vector a + i*b; after the FFT the half spectra of
 // both are separated with Z(k) and conj(Z(N-k)).  Columns beyond W repeat column W-1 for kX
-// samples, then column 0 (padDataClampToBorder, convolutionFFT2D.cuh:56-85).
+// samples, then column 0 (padDataClampToBorder, convolutionFFT2D.cuh:56-85).  blockIdx.y
+// selects the scale of the batch (scales of one pyramid level differ only in kX).
 template <int FW>
-__global__ void __launch_bounds__(256) k_fwd_rows(const uint8_t* __restrict__ img, size_t pitch,
-                                                  int H, int W, int kX, float scale,
-                                                  float2* __restrict__ R, int ldr,
-                                                  const float2* __restrict__ tw) {
+__global__ void __launch_bounds__(64) k_fwd_rows(const uint8_t* __restrict__ img, size_t pitch,
+                                                 int H, int W, ScaleSet ss, float scale,
+                                                 float2* __restrict__ Rbase, int ldr,
+                                                 size_t rstride, const float2* __restrict__ tw) {
     using C = FFTCfg<FW>;
-    extern __shared__ float2 lds[];
+    extern __shared__ float lds[];
+    const int sidx = blockIdx.y;
+    const int kX = ss.kX[sidx];
+    float2* R = Rbase + (size_t)sidx * rstride;
     const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
-    const int vpb = blockDim.x / C::TPV;
-    const int ya = 2 * (blockIdx.x * vpb + g), yb = ya + 1;
+    const int ya = 2 * (blockIdx.x * RowCfg<FW>::VPB + g), yb = ya + 1;
     const bool haveA = ya < H, haveB = yb < H;
-    float2* my = lds + (size_t)g * FW;
+    float* my = lds + (size_t)g * 2 * FW;
     const uint8_t* rowA = img + (size_t)(haveA ? ya : 0) * pitch;
     const uint8_t* rowB = img + (size_t)(haveB ? yb : 0) * pitch;
     float2 v[C::E];
@@ -77,12 +91,13 @@ __global__ void __launch_bounds__(256) k_fwd_rows(const uint8_t* __restrict__ im
         v[e] = make_float2(a, b);
     }
     fft<FW, false>(v, my, lane, tw);
+    float2* z = reinterpret_cast<float2*>(my);
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) my[out_index<FW>(e, lane)] = v[e];
-    __syncthreads();
+    for (int e = 0; e < C::E; ++e) z[out_index<FW>(e, lane)] = v[e];
+    stage_sync();
     for (int k = lane; k <= FW / 2; k += C::TPV) {
-        const float2 z1 = my[k];
-        const float2 z2 = my[k == 0 ? 0 : FW - k];
+        const float2 z1 = z[k];
+        const float2 z2 = z[k == 0 ? 0 : FW - k];
         if (haveA) R[(size_t)ya * ldr + k] = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
         if (haveB) R[(size_t)yb * ldr + k] = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
     }
@@ -102,29 +117,33 @@ struct ColCfg {
 
 // Column FFT of the row spectra R (H rows, ldr complex per row) with clamp padding in y:
 // rows [H, H+kY) repeat row H-1, later rows repeat row 0.  Output column kx is written
-// contiguously at DT + kx*FH.
+// contiguously at DT + kx*FH.  blockIdx.y selects the scale of the batch.
 template <int FH>
 __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
-        const float2* __restrict__ R, int ldr, int H, int kY, int ncols,
-        float2* __restrict__ DT, const float2* __restrict__ tw) {
+        const float2* __restrict__ Rbase, int ldr, size_t rstride, int H, ScaleSet ss, int ncols,
+        const float2* __restrict__ tw) {
     using C = FFTCfg<FH>;
     using CC = ColCfg<FH>;
-    extern __shared__ float2 lds[];
+    extern __shared__ float2 lds2[];
+    const int sidx = blockIdx.y;
+    const int kY = ss.kX[sidx];
+    const float2* R = Rbase + (size_t)sidx * rstride;
+    float2* DT = ss.DT[sidx];
     const int kx0 = blockIdx.x * CC::CB;
     for (int t = threadIdx.x; t < FH * CC::CB; t += CC::THREADS) {
         const int b = t % CC::CB, y = t / CC::CB;
         const int kx = kx0 + b;
         const int ry = y < H ? y : (y < H + kY ? H - 1 : 0);
-        lds[b * CC::S + y] = kx < ncols ? R[(size_t)ry * ldr + kx] : make_float2(0.f, 0.f);
+        lds2[b * CC::S + y] = kx < ncols ? R[(size_t)ry * ldr + kx] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
-    float2* my = lds + (size_t)g * CC::S;
+    float2* my = lds2 + (size_t)g * CC::S;
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = my[in_index<FH>(e, lane)];
-    __syncthreads();
-    fft<FH, false>(v, my, lane, tw);
+    stage_sync();
+    fft<FH, false>(v, reinterpret_cast<float*>(my), lane, tw);
     const int kx = kx0 + g;
     if (kx < ncols) {
         float2* out = DT + (size_t)kx * FH;
@@ -133,78 +152,120 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
     }
 }
 
-// Inverse column FFT of Z_o = D * G_o for orientation o = blockIdx.y.  D holds columns
-// kx <= FW/2; column kx > FW/2 is conj(D(-ky, FW-kx)) (Hermitian symmetry of a real image).
-// The bank stores orientations 0..4; G_o for o = 5..7 is G_{8-o} mirrored in kx.  Rows
-// y < Hc of the result are written row-major into T (Hc x FW per orientation).
+// Inverse column pass.  A workgroup owns one orientation o and CB stored spectrum columns
+// kx in [k0, k0+CB) (kx <= FW/2) plus the Hermitian partners FW-kx of the columns
+// [k0+1, k0+CB+1), so that both output blocks start on a CB-aligned column.  A partner
+// column's spectrum is conj(D(-ky, kx)); its inverse transform is taken as
+// conj(IFFT[D(ky, kx) * G(-ky, FW-kx)]) (G is real).  The bank stores orientations 0..4; G_o
+// for o = 5..7 is G_{8-o} mirrored in kx.  Workgroups are dealt round-robin to the 8 XCDs;
+// the numbering puts the eight orientations of one column block back to back on one XCD, so
+// D and the shared bank columns come from HBM once and from that XCD's L2 afterwards.  Rows
+// y < Hc go row-major into T (Hc x FW per orientation) through LDS, half at a time.
+template <int FH>
+struct InvColCfg {
+    static constexpr int CB = ColCfg<FH>::CB;
+    static constexpr int THREADS = ColCfg<FH>::THREADS;
+    // rows leave through LDS in the blocks of the last radix: register slot e holds row
+    // lane + TPV*(e/RL) + (e%RL)*BR, so block membership is known at compile time
+    static constexpr int RL = RadInfo<typename FFTCfg<FH>::Rad>::last;
+    static constexpr int BR = FH / RL;
+    static constexpr int SS = BR + 8;  // staging stride (float2); == 8 (mod 32) when 32 | BR
+    static constexpr size_t STAGE = (size_t)CB * SS * sizeof(float2);
+    static constexpr size_t EXCH = (size_t)CB * FH * sizeof(float);
+    static constexpr size_t LDS = STAGE > EXCH ? STAGE : EXCH;
+};
+
 template <int FH>
 __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
-        const float2* __restrict__ DT, const float* __restrict__ G, int FW, int Hc,
-        float2* __restrict__ T, const float2* __restrict__ tw) {
+        ScaleSet ss, int FW, int Hc, int nblk, float2* __restrict__ Tbase, size_t tstride,
+        const float2* __restrict__ tw) {
     using C = FFTCfg<FH>;
-    using CC = ColCfg<FH>;
-    extern __shared__ float2 lds[];
-    const int o = blockIdx.y;
-    const int kx0 = blockIdx.x * CC::CB;
-    const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
-    const int kx = kx0 + g;
-    const bool mirror = kx > FW / 2;
-    const float2* dcol = DT + (size_t)(mirror ? FW - kx : kx) * FH;
+    using IC = InvColCfg<FH>;
+    constexpr int CB = IC::CB;
+    extern __shared__ float2 lds2[];
+    // XCD-aware numbering: the i-th workgroup of XCD x handles orientation i % 8 of column
+    // block 8*(i/8) + x (column blocks of all scales of the batch concatenated)
+    const int xcd = blockIdx.x % 8, i = blockIdx.x / 8;
+    const int o = i % kNumOri, gcb = (i / kNumOri) * 8 + xcd;
+    if (gcb >= nblk * ss.n) return;  // uniform per workgroup: no barrier is skipped
+    const int sidx = gcb / nblk, k0 = (gcb % nblk) * CB;
+    const float2* DT = ss.DT[sidx];
+    const float* G = ss.G[sidx];
+    float2* To = Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW;
+    const int half = FW / 2;
     const int og = o <= 4 ? o : kNumOri - o;
-    const int kxg = (o <= 4 || kx == 0) ? kx : FW - kx;
-    const float* gcol = G + ((size_t)og * FW + kxg) * FH;
-    float2 v[C::E];
+    const int g = threadIdx.x / C::TPV, lane0 = threadIdx.x % C::TPV;
+    float* my = reinterpret_cast<float*>(lds2) + (size_t)g * FH;
+
+#pragma unroll 1
+    for (int partner = 0; partner < 2; ++partner) {
+        // lane-derived indices are recomputed per pass rather than kept live across passes
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        // direct: column kx = k0 + g; partner: stored column kx = k0 + 1 + g, output FW - kx
+        const int kx = k0 + partner + g;
+        const bool ok = partner ? (kx < half) : (kx <= half);
+        const float2* dcol = DT + (size_t)(ok ? kx : 0) * FH;
+        // bank column: direct o <= 4 -> kx, o >= 5 -> FW-kx; partner o <= 4 -> FW-kx, else kx
+        const int cg = ((o <= 4) != (partner != 0)) ? kx : (kx == 0 ? 0 : FW - kx);
+        const float* gcol = G + ((size_t)og * FW + (ok ? cg : 0)) * FH;
+        float2 v[C::E];
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) {
-        const int ky = in_index<FH>(e, lane);
-        float2 d;
-        if (mirror) {
-            const float2 t = dcol[ky == 0 ? 0 : FH - ky];
-            d = make_float2(t.x, -t.y);
-        } else {
-            d = dcol[ky];
+        for (int e = 0; e < C::E; ++e) {
+            const int ky = in_index<FH>(e, lane);
+            const int kg = (partner && ky) ? FH - ky : ky;
+            v[e] = cscale(dcol[ky], gcol[kg]);
         }
-        v[e] = cscale(d, gcol[ky]);
-    }
-    float2* my = lds + (size_t)g * CC::S;
-    fft<FH, true>(v, my, lane, tw);
+        fft<FH, true>(v, my, lane, tw);
+        if (partner) {
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) my[out_index<FH>(e, lane)] = v[e];
-    __syncthreads();
-    float2* To = T + (size_t)o * Hc * FW;
-    for (int t = threadIdx.x; t < Hc * CC::CB; t += CC::THREADS) {
-        const int b = t % CC::CB, y = t / CC::CB;
-        To[(size_t)y * FW + kx0 + b] = lds[b * CC::S + y];
+            for (int e = 0; e < C::E; ++e) v[e].y = -v[e].y;
+        }
+        // output column of staging slot b: direct k0 + b; partner FW - (k0 + 1 + (CB-1-b))
+        const int slot = partner ? CB - 1 - g : g;
+        const int base = partner ? FW - k0 - CB : k0;
+#pragma unroll
+        for (int rb = 0; rb < IC::RL; ++rb) {
+            const int ylo = rb * IC::BR;
+            if (ylo >= Hc) break;  // uniform
+            const int rows = min(Hc - ylo, IC::BR);
+            __syncthreads();  // transforms and earlier readers of the region are done
+#pragma unroll
+            for (int m = 0; m < C::E / IC::RL; ++m)
+                lds2[slot * IC::SS + lane + C::TPV * m] = v[m * IC::RL + rb];
+            __syncthreads();
+            for (int t = threadIdx.x; t < rows * CB; t += IC::THREADS) {
+                const int b = t % CB, yy = t / CB;
+                const int ks = partner ? k0 + CB - b : k0 + b;  // stored column
+                const bool w = partner ? (ks < half) : (ks <= half);
+                if (w) To[(size_t)(ylo + yy) * FW + base + b] = lds2[b * IC::SS + yy];
+            }
+        }
+        __syncthreads();  // the exchange region is reused by the next transform
     }
 }
 
 // ------------------------------------------------------------------ inverse rows
-template <int N>
-struct RowCfg {
-    static constexpr int TPV = FFTCfg<N>::TPV;
-    static constexpr int THREADS = 256;
-    static constexpr int VPB = THREADS / TPV;
-};
-
 // Inverse row FFT of T_o row y and complex-cell magnitude sqrt(e^2 + o^2) for x < Wc
-// (complexResponse_kernel, cuda_bimp.cuh:165-180).
+// (complexResponse_kernel, cuda_bimp.cuh:165-180).  blockIdx.y = orientation,
+// blockIdx.z = scale of the batch.
 template <int FW>
-__global__ void __launch_bounds__(256) k_inv_rows(const float2* __restrict__ T, int Hc, int Wc,
-                                                  float* __restrict__ Cm, int ldc,
-                                                  const float2* __restrict__ tw) {
+__global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __restrict__ Tbase,
+                                                 size_t tstride, int Hc, int Wc, int ldc,
+                                                 const float2* __restrict__ tw) {
     using C = FFTCfg<FW>;
-    extern __shared__ float2 lds[];
-    const int o = blockIdx.y;
+    extern __shared__ float lds[];
+    const int o = blockIdx.y, sidx = blockIdx.z;
     const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
     const int y = blockIdx.x * RowCfg<FW>::VPB + g;
     const bool active = y < Hc;
-    const float2* row = T + ((size_t)o * Hc + (active ? y : 0)) * FW;
+    const float2* row = Tbase + (size_t)sidx * tstride + ((size_t)o * Hc + (active ? y : 0)) * FW;
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = row[in_index<FW>(e, lane)];
     fft<FW, true>(v, lds + (size_t)g * FW, lane, tw);
     if (active) {
-        float* out = Cm + ((size_t)o * Hc + y) * ldc;
+        float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
 #pragma unroll
         for (int e = 0; e < C::E; ++e) {
             const int x = out_index<FW>(e, lane);
@@ -244,66 +305,257 @@ struct MapView {
     }
 };
 
-__global__ void __launch_bounds__(256) k_cells(MapView mv, int H, int W, CellParams p,
-                                               float* __restrict__ Dm, float* __restrict__ Sm,
-                                               unsigned* __restrict__ maxslot) {
+// Output tile of the cells kernel: 64 x 16 pixels, 512 threads, 2 rows per thread.  The
+// radial term couples orientation o with o+4, so the maps are staged in LDS in the pairs
+// (o, o+4), o = 0..3, each tile with a halo of HS pixels (HS >= twice the largest offset, a
+// multiple of 4 so that the staging moves aligned float4).  While one pair is evaluated the
+// next pair is already in flight into registers.  The contributions of o+4 wait in
+// registers so that every sum still runs in the order o = 0..7 of the reference's serialised
+// launches; an invalid orientation contributes +0, which leaves a sum of non-negative terms
+// unchanged (so does a -0 from the clamps).
+constexpr int kCellTW = 64, kCellTH = 16, kCellTY = 8, kCellRows = kCellTH / kCellTY;
+// dynamic LDS limit per workgroup (160 KiB per CU, minus room for static LDS)
+constexpr size_t kMaxLds = 160 * 1024 - 1024;
+
+struct CellTileOffsets {
+    // LDS element offsets relative to the pixel's own position in a staged tile
+    int es1[kNumOri], es2[kNumOri], ed1[kNumOri], ed2[kNumOri];  // (-a,b) (a,-b) (-2a,2b) (2a,-2b)
+    int tn[kNumOri];                                              // (-a',-b')
+    int rp[kNumOri], rm[kNumOri];                                 // partner map at +-(a'',b'')
+    // validity: 2|b| <= x <= W - 2|b| and 2|a| <= y <= H - 2|a| (end-stop, inhibition)
+    int ex[kNumOri], ey[kNumOri], ix[kNumOri], iy[kNumOri];
+};
+
+struct Contrib {
+    float s, d, t, r;
+};
+
+// Contributions of orientation o at one pixel: m = its own tile, m2 = the tile of (o+4)%8.
+__device__ __forceinline__ Contrib cell_terms(const float* __restrict__ m,
+                                              const float* __restrict__ m2,
+                                              const CellTileOffsets& q, int o) {
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const bool in = x < W && y < H;
-    float Dv = -INFINITY;
-    if (in) {
-        float S = 0.f, D = 0.f, Tn = 0.f, Rd = 0.f;
+    Contrib c;
+    const float centre = m[0];
+    c.s = fmaxf(m[q.es1[o]] - m[q.es2[o]], 0.f);
+    c.d = fmaxf((float)((double)centre - 0.5 * (double)m[q.ed1[o]] -
+                        0.5 * (double)m[q.ed2[o]]),
+                0.f);
+    c.t = fmaxf(m[q.tn[o]] - centre, 0.f);
+    float v = fmaxf(centre - 16.f * m2[q.rp[o]], 0.f);
+    v += centre - 16.f * m2[q.rm[o]];
+    c.r = fmaxf(v, 0.f);
+    return c;
+}
+
+template <int HS>
+struct CellGeom {
+    static constexpr int LW = kCellTW + 2 * HS, LH = kCellTH + 2 * HS, TS = LW * LH;
+    static constexpr int LW4 = LW / 4;
+    static constexpr int NV4 = 2 * LH * LW4;                       // float4 per staged pair
+    static constexpr int PER = (NV4 + kCellTW * kCellTY - 1) / (kCellTW * kCellTY);
+};
+
+// Loads (INTERIOR: directly; else with the linear-index semantics and zero outside
+// [0,H] x [0,W]) the float4s of pair pr that this thread stages.
+template <int HS, bool INTERIOR>
+__device__ __forceinline__ void cells_fetch(const MapView& mv, int H, int W, int x0, int y0,
+                                            int pr, int tid, float4 (&r)[CellGeom<HS>::PER]) {
+    using CG = CellGeom<HS>;
 #pragma unroll
-        for (int o = 0; o < kNumOri; ++o) {
-            const int a = p.ea[o], b = p.eb[o];
-            if (!stencil_valid(x, y, a, b, W, H)) continue;
-            float v = mv.at(o, y - a, x + b) - mv.at(o, y + a, x - b);
-            v = v > 0.f ? v : 0.f;
-            if (v > 0.f) S += v;
-            v = (float)((double)mv.at(o, y, x) - 0.5 * (double)mv.at(o, y - 2 * a, x + 2 * b) -
-                        0.5 * (double)mv.at(o, y + 2 * a, x - 2 * b));
-            v = v > 0.f ? v : 0.f;
-            if (v > 0.f) D += v;
-        }
+    for (int k = 0; k < CG::PER; ++k) {
+        const int i = tid + k * kCellTW * kCellTY;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < CG::NV4) {
+            const int row = i / CG::LW4, c4 = i - row * CG::LW4;
+            const int m = row >= CG::LH ? 1 : 0;
+            const int ry = row - m * CG::LH;
+            const int o = pr + 4 * m;
+            const int gy = y0 - HS + ry, gx = x0 - HS + 4 * c4;
+            if (INTERIOR) {
+                v = *reinterpret_cast<const float4*>(mv.base + ((size_t)o * mv.Hc + gy) * mv.ldc + gx);
+            } else {
+                float e[4];
 #pragma unroll
-        for (int o = 0; o < kNumOri; ++o) {
-            const int a = p.ia[o], b = p.ib[o];
-            if (!stencil_valid(x, y, a, b, W, H)) continue;
-            const int o2 = (o + kNumOri / 2) % kNumOri;
-            const float centre = mv.at(o, y, x);
-            float v = mv.at(o, y - a, x - b) - centre;
-            v = v < 0.f ? 0.f : v;
-            Tn += v;
-            const int a2 = p.ra[o], b2 = p.rb[o];
-            v = centre - 16.f * mv.at(o2, y + a2, x + b2);
-            v = v < 0.f ? 0.f : v;
-            v += centre - 16.f * mv.at(o2, y - a2, x - b2);
-            v = v < 0.f ? 0.f : v;
-            Rd += v;
+                for (int j = 0; j < 4; ++j)
+                    e[j] = (gy >= 0 && gy <= H && gx + j >= 0 && gx + j <= W) ? mv.at(o, gy, gx + j)
+                                                                                : 0.f;
+                v = make_float4(e[0], e[1], e[2], e[3]);
+            }
         }
-        const float inhib = Tn + Rd;
-        Dv = D + (-inhib);
-        Dm[(size_t)y * W + x] = Dv;
-        Sm[(size_t)y * W + x] = S + (-inhib);
+        r[k] = v;
+    }
+}
+
+template <int HS>
+__device__ __forceinline__ void cells_put(float* __restrict__ tile, int tid,
+                                          const float4 (&r)[CellGeom<HS>::PER]) {
+    using CG = CellGeom<HS>;
+#pragma unroll
+    for (int k = 0; k < CG::PER; ++k) {
+        const int i = tid + k * kCellTW * kCellTY;
+        if (i < CG::NV4) reinterpret_cast<float4*>(tile)[i] = r[k];
+    }
+}
+
+// INTERIOR: every pixel of the tile is valid for every orientation and the staged tile with
+// its halo lies inside [0,H] x [0,W] without wrap-around, so no test is needed anywhere.
+template <int HS, bool INTERIOR>
+__device__ __forceinline__ void cells_tile(const MapView& mv, int H, int W,
+                                           const CellTileOffsets& q, int x0, int y0,
+                                           float* __restrict__ tile, float* __restrict__ Dm,
+                                           float* __restrict__ Sm, float& Dmax, bool& any) {
+#pragma clang fp contract(off)
+    using CG = CellGeom<HS>;
+    const int tid = threadIdx.y * kCellTW + threadIdx.x;
+    const int x = x0 + threadIdx.x;
+    float accS[kCellRows], accD[kCellRows], accT[kCellRows], accR[kCellRows];
+    Contrib hold[kCellRows][4];
+#pragma unroll
+    for (int rr = 0; rr < kCellRows; ++rr) accS[rr] = accD[rr] = accT[rr] = accR[rr] = 0.f;
+    float4 pre[CG::PER];
+    cells_fetch<HS, INTERIOR>(mv, H, W, x0, y0, 0, tid, pre);
+    cells_put<HS>(tile, tid, pre);
+    __syncthreads();
+#pragma unroll
+    for (int pr = 0; pr < 4; ++pr) {
+        if (pr < 3) cells_fetch<HS, INTERIOR>(mv, H, W, x0, y0, pr + 1, tid, pre);
+        const int o1 = pr, o2 = pr + 4;
+#pragma unroll
+        for (int rr = 0; rr < kCellRows; ++rr) {
+            const int y = y0 + threadIdx.y + kCellTY * rr;
+            const int li = (y - y0 + HS) * CG::LW + (x - x0 + HS);
+            const float* m1 = tile + li;
+            const float* m2 = tile + CG::TS + li;
+            Contrib c1 = cell_terms(m1, m2, q, o1);
+            Contrib c2 = cell_terms(m2, m1, q, o2);
+            if (!INTERIOR) {
+                const bool e1 = x >= q.ex[o1] && x <= W - q.ex[o1] && y >= q.ey[o1] && y <= H - q.ey[o1];
+                const bool i1 = x >= q.ix[o1] && x <= W - q.ix[o1] && y >= q.iy[o1] && y <= H - q.iy[o1];
+                const bool e2 = x >= q.ex[o2] && x <= W - q.ex[o2] && y >= q.ey[o2] && y <= H - q.ey[o2];
+                const bool i2 = x >= q.ix[o2] && x <= W - q.ix[o2] && y >= q.iy[o2] && y <= H - q.iy[o2];
+                if (!e1) c1.s = c1.d = 0.f;
+                if (!i1) c1.t = c1.r = 0.f;
+                if (!e2) c2.s = c2.d = 0.f;
+                if (!i2) c2.t = c2.r = 0.f;
+            }
+            hold[rr][pr] = c2;
+            accS[rr] += c1.s;
+            accD[rr] += c1.d;
+            accT[rr] += c1.t;
+            accR[rr] += c1.r;
+        }
+        if (pr < 3) {
+            __syncthreads();
+            cells_put<HS>(tile, tid, pre);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int rr = 0; rr < kCellRows; ++rr) {
+        const int y = y0 + threadIdx.y + kCellTY * rr;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            accS[rr] += hold[rr][k].s;
+            accD[rr] += hold[rr][k].d;
+            accT[rr] += hold[rr][k].t;
+            accR[rr] += hold[rr][k].r;
+        }
+        if (INTERIOR || (x < W && y < H)) {
+            const float inhib = accT[rr] + accR[rr];
+            const float Dv = accD[rr] + (-inhib);
+            Dm[(size_t)y * W + x] = Dv;
+            Sm[(size_t)y * W + x] = accS[rr] + (-inhib);
+            Dmax = any ? fmaxf(Dmax, Dv) : Dv;
+            any = true;
+        }
+    }
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs; tiles are renumbered so that each XCD
+// walks a contiguous raster band and the halos it re-reads stay in its own L2.
+__device__ __forceinline__ int xcd_band_index(int b, int n) {
+    const int per = (n + 7) / 8;
+    const int t = (b % 8) * per + b / 8;
+    return t < n ? t : -1;
+}
+
+// Tile geometry of one scale: tiles_x x tiles_y tiles, the interior ones (INTERIOR above)
+// forming the rectangle [ix0, ix1) x [iy0, iy1).
+struct CellTiles {
+    int tiles_x, tiles_y, ix0, ix1, iy0, iy1;
+    __device__ __forceinline__ int n_interior() const { return (ix1 - ix0) * (iy1 - iy0); }
+    __device__ __forceinline__ int n_border() const {
+        return tiles_x * tiles_y - n_interior();
+    }
+    // k-th border tile: rows above, rows below, then the left/right columns in between
+    __device__ __forceinline__ void border_tile(int k, int& tx, int& ty) const {
+        const int ntop = iy0 * tiles_x, nbot = (tiles_y - iy1) * tiles_x;
+        if (k < ntop) {
+            tx = k % tiles_x;
+            ty = k / tiles_x;
+        } else if (k < ntop + nbot) {
+            k -= ntop;
+            tx = k % tiles_x;
+            ty = iy1 + k / tiles_x;
+        } else {
+            k -= ntop + nbot;
+            const int side = ix0 + (tiles_x - ix1);
+            ty = iy0 + k / side;
+            const int c = k % side;
+            tx = c < ix0 ? c : ix1 + (c - ix0);
+        }
+    }
+};
+
+// One launch per scale: the first n_border workgroups take the border tiles (checked path),
+// the rest the interior rectangle (unchecked path, XCD-banded).
+template <int HS>
+__global__ void __launch_bounds__(512) k_cells(MapView mv, int H, int W, CellTileOffsets q,
+                                               CellTiles tg, float* __restrict__ Dm,
+                                               float* __restrict__ Sm,
+                                               unsigned* __restrict__ maxslot) {
+    extern __shared__ float4 tile4[];
+    float* tile = reinterpret_cast<float*>(tile4);
+    const int tid = threadIdx.y * kCellTW + threadIdx.x;
+    float Dmax = -INFINITY;
+    bool any = false;
+    const int nb = tg.n_border();
+    if ((int)blockIdx.x < nb) {
+        int tx, ty;
+        tg.border_tile(blockIdx.x, tx, ty);
+        cells_tile<HS, false>(mv, H, W, q, tx * kCellTW, ty * kCellTH, tile, Dm, Sm, Dmax, any);
+    } else {
+        const int t = xcd_band_index(blockIdx.x - nb, gridDim.x - nb);
+        const int nx = tg.ix1 - tg.ix0;
+        if (t >= 0 && t < tg.n_interior())
+            cells_tile<HS, true>(mv, H, W, q, (tg.ix0 + t % nx) * kCellTW,
+                                 (tg.iy0 + t / nx) * kCellTH, tile, Dm, Sm, Dmax, any);
     }
     // block maximum of D -> one atomic per workgroup
-    unsigned key = in ? float_key(Dv) : 0u;
+    unsigned key = any ? float_key(Dmax) : 0u;
 #pragma unroll
     for (int sh = 32; sh >= 1; sh >>= 1) {
         const unsigned other = __shfl_xor(key, sh, 64);
         key = other > key ? other : key;
     }
-    __shared__ unsigned wmax[16];
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    __shared__ unsigned wmax[kCellTY];
     if ((tid & 63) == 0) wmax[tid >> 6] = key;
     __syncthreads();
     if (tid == 0) {
         unsigned m = 0;
-        for (int i = 0; i < nw; ++i) m = wmax[i] > m ? wmax[i] : m;
+        for (int i = 0; i < kCellTY; ++i) m = wmax[i] > m ? wmax[i] : m;
         if (m) atomicMax(maxslot, m);
     }
+}
+
+int cells_halo(const CellParams& p) {
+    int h = 0;
+    for (int o = 0; o < kNumOri; ++o) {
+        h = max(h, 2 * max(abs(p.ea[o]), abs(p.eb[o])));
+        h = max(h, 2 * max(abs(p.ia[o]), abs(p.ib[o])));
+    }
+    return h;
 }
 
 // ------------------------------------------------------------------ NMS + compaction
@@ -506,7 +758,45 @@ __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, s
 }  // namespace
 
 // ------------------------------------------------------------------ dispatch
+#define BIMP_CELL_HALOS(X) X(4) X(8) X(12) X(16) X(20)
 #define BIMP_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(4096)
+
+// Host construction of the packed twiddle table of FFTCfg<N> (layout: TwLayout).
+template <class L, int Ns>
+struct TwBuild;
+template <int R, int... Rest, int Ns>
+struct TwBuild<RadList<R, Rest...>, Ns> {
+    static void run(std::vector<float2>& out, bool first) {
+        if (!first) {
+            for (int k = 0; k < Ns; ++k) {
+                for (int r = 1; r < R; ++r) {
+                    const double a = -2.0 * 3.14159265358979323846 * (double)r * k / ((double)Ns * R);
+                    out.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+                }
+                for (int r = R - 1; r < tw_row(R); ++r) out.push_back(make_float2(0.f, 0.f));
+            }
+        }
+        TwBuild<RadList<Rest...>, Ns * R>::run(out, false);
+    }
+};
+template <int Ns>
+struct TwBuild<RadList<>, Ns> {
+    static void run(std::vector<float2>&, bool) {}
+};
+
+template <int N>
+static void twiddles_n(std::vector<float2>& out) {
+    out.clear();
+    TwBuild<typename FFTCfg<N>::Rad, 1>::run(out, true);
+    if (out.empty()) out.push_back(make_float2(1.f, 0.f));
+}
+
+bool twiddle_table(int n, std::vector<float2>& out) {
+#define X(N) if (n == N) { twiddles_n<N>(out); return true; }
+    BIMP_FOR_SIZES(X)
+#undef X
+    return false;
+}
 
 bool fft_size_supported(int n) {
 #define X(N) if (n == N) return true;
@@ -532,7 +822,7 @@ int row_vectors_per_block(int n) {
 template <int N>
 static hipError_t prepare_n() {
     hipError_t e;
-    const int rowLds = RowCfg<N>::VPB * N * (int)sizeof(float2);
+    const int rowLds = RowCfg<N>::VPB * N * (int)sizeof(float2);  // fwd rows stage complex output
     const int colLds = ColCfg<N>::CB * ColCfg<N>::S * (int)sizeof(float2);
     if ((e = hipFuncSetAttribute((const void*)k_fwd_rows<N>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, rowLds)) != hipSuccess)
@@ -544,7 +834,18 @@ static hipError_t prepare_n() {
                                  hipFuncAttributeMaxDynamicSharedMemorySize, colLds)) != hipSuccess)
         return e;
     return hipFuncSetAttribute((const void*)k_inv_cols<N>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, colLds);
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+}
+
+hipError_t prepare_common() {
+    hipError_t e = hipSuccess;
+#define X(HS)                                                                                \
+    if (e == hipSuccess)                                                                     \
+        e = hipFuncSetAttribute((const void*)k_cells<HS>,                                    \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    BIMP_CELL_HALOS(X)
+#undef X
+    return e;
 }
 
 hipError_t prepare_size(int n) {
@@ -566,14 +867,16 @@ hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, ui
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W, int kX,
-                           float scale, float2* R, int ldr, const float2* tw, hipStream_t s) {
+hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W,
+                           const ScaleSet& ss, float scale, float2* R, int ldr, size_t rstride,
+                           const float2* tw, hipStream_t s) {
 #define X(N)                                                                                 \
     if (FW == N) {                                                                           \
         const int vpb = RowCfg<N>::VPB;                                                      \
         const int pairs = (H + 1) / 2;                                                       \
-        k_fwd_rows<N><<<(pairs + vpb - 1) / vpb, 256, vpb * N * sizeof(float2), s>>>(        \
-            img, pitch, H, W, kX, scale, R, ldr, tw);                                        \
+        dim3 grid((pairs + vpb - 1) / vpb, ss.n);                                            \
+        k_fwd_rows<N><<<grid, RowCfg<N>::THREADS, vpb * N * sizeof(float2), s>>>(            \
+            img, pitch, H, W, ss, scale, R, ldr, rstride, tw);                               \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
@@ -581,13 +884,14 @@ hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, int H, int kY, int ncols,
-                           float2* DT, const float2* tw, hipStream_t s) {
+hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, size_t rstride, int H,
+                           const ScaleSet& ss, int ncols, const float2* tw, hipStream_t s) {
 #define X(N)                                                                                 \
     if (FH == N) {                                                                           \
         using CC = ColCfg<N>;                                                                \
-        k_fwd_cols<N><<<(ncols + CC::CB - 1) / CC::CB, CC::THREADS,                          \
-                        CC::CB * CC::S * sizeof(float2), s>>>(R, ldr, H, kY, ncols, DT, tw); \
+        dim3 grid((ncols + CC::CB - 1) / CC::CB, ss.n);                                      \
+        k_fwd_cols<N><<<grid, CC::THREADS, CC::CB * CC::S * sizeof(float2), s>>>(            \
+            R, ldr, rstride, H, ss, ncols, tw);                                              \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
@@ -595,15 +899,17 @@ hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, int H, int kY, int 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inv_cols(int FH, const float2* DT, const float* G, int FW, int Hc, float2* T,
-                           const float2* tw, hipStream_t s) {
+hipError_t launch_inv_cols(int FH, const ScaleSet& ss, int FW, int Hc, float2* T,
+                           size_t tstride, const float2* tw, hipStream_t s) {
 #define X(N)                                                                                 \
     if (FH == N) {                                                                           \
-        using CC = ColCfg<N>;                                                                \
-        if (FW % CC::CB) return hipErrorInvalidValue;                                        \
-        dim3 grid(FW / CC::CB, kNumOri);                                                     \
-        k_inv_cols<N><<<grid, CC::THREADS, CC::CB * CC::S * sizeof(float2), s>>>(            \
-            DT, G, FW, Hc, T, tw);                                                           \
+        using IC = InvColCfg<N>;                                                             \
+        const size_t lds = IC::LDS;                                                          \
+        if (lds > kMaxLds || Hc > N || FW % IC::CB) return hipErrorInvalidValue;            \
+        const int nblk = (FW / 2 + IC::CB) / IC::CB;                                         \
+        const int groups = (nblk * ss.n + 7) / 8;                                            \
+        k_inv_cols<N><<<groups * 8 * kNumOri, IC::THREADS, lds, s>>>(ss, FW, Hc, nblk, T,    \
+                                                                      tstride, tw);          \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
@@ -611,28 +917,73 @@ hipError_t launch_inv_cols(int FH, const float2* DT, const float* G, int FW, int
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inv_rows(int FW, const float2* T, int Hc, int Wc, float* C, int ldc,
-                           const float2* tw, hipStream_t s) {
+hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t tstride, int Hc,
+                           int Wc, int ldc, const float2* tw, hipStream_t s) {
 #define X(N)                                                                                 \
     if (FW == N) {                                                                           \
         const int vpb = RowCfg<N>::VPB;                                                      \
-        dim3 grid((Hc + vpb - 1) / vpb, kNumOri);                                            \
-        k_inv_rows<N><<<grid, 256, vpb * N * sizeof(float2), s>>>(T, Hc, Wc, C, ldc, tw);    \
+        dim3 grid((Hc + vpb - 1) / vpb, kNumOri, ss.n);                                      \
+        k_inv_rows<N><<<grid, RowCfg<N>::THREADS, vpb * N * sizeof(float), s>>>(             \
+            ss, T, tstride, Hc, Wc, ldc, tw);                                                \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+template <int HS>
+static hipError_t launch_cells_hs(const MapView& mv, int FH, int FW, int H, int W,
+                                  const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
+                                  hipStream_t s) {
+    using CG = CellGeom<HS>;
+    const size_t lds = (size_t)2 * CG::TS * sizeof(float);
+    CellTileOffsets q;
+    for (int o = 0; o < kNumOri; ++o) {
+        const int a = p.ea[o], b = p.eb[o];
+        q.es1[o] = -a * CG::LW + b;
+        q.es2[o] = a * CG::LW - b;
+        q.ed1[o] = -2 * a * CG::LW + 2 * b;
+        q.ed2[o] = 2 * a * CG::LW - 2 * b;
+        q.tn[o] = -p.ia[o] * CG::LW - p.ib[o];
+        q.rp[o] = p.ra[o] * CG::LW + p.rb[o];
+        q.rm[o] = -p.ra[o] * CG::LW - p.rb[o];
+        q.ex[o] = 2 * abs(b);
+        q.ey[o] = 2 * abs(a);
+        q.ix[o] = 2 * abs(p.ib[o]);
+        q.iy[o] = 2 * abs(p.ia[o]);
+    }
+    dim3 block(kCellTW, kCellTY);
+    CellTiles tg;
+    tg.tiles_x = (W + kCellTW - 1) / kCellTW;
+    tg.tiles_y = (H + kCellTH - 1) / kCellTH;
+    // interior tiles: x0 >= HS, x0 + TW - 1 <= W - HS (same in y), no wrap-around reads
+    auto first = [](int hh, int t) { return (hh + t - 1) / t; };
+    auto end = [](int lim, int t) { return lim >= 0 ? lim / t + 1 : 0; };  // x0 <= lim
+    tg.ix0 = first(HS, kCellTW);
+    tg.ix1 = std::max(tg.ix0, std::min(tg.tiles_x, end(W - HS - (kCellTW - 1), kCellTW)));
+    tg.iy0 = first(HS, kCellTH);
+    tg.iy1 = std::max(tg.iy0, std::min(tg.tiles_y, end(H - HS - (kCellTH - 1), kCellTH)));
+    if (W >= FW || H >= FH || tg.ix0 >= tg.tiles_x || tg.iy0 >= tg.tiles_y)
+        tg.ix0 = tg.ix1 = tg.iy0 = tg.iy1 = 0;  // everything takes the checked path
+    const int nin = (tg.ix1 - tg.ix0) * (tg.iy1 - tg.iy0);
+    const int nbd = tg.tiles_x * tg.tiles_y - nin;
+    const int grid = nbd + (nin + 7) / 8 * 8;
+    k_cells<HS><<<grid, block, lds, s>>>(mv, H, W, q, tg, Dm, Sm, maxslot);
+    return hipGetLastError();
 }
 
 hipError_t launch_cells(const float* C, int Hc, int ldc, int FH, int FW, int H, int W,
                         const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
                         hipStream_t s) {
     MapView mv{C, Hc, ldc, FH, FW};
-    dim3 block(64, 4);
-    dim3 grid((W + 63) / 64, (H + 3) / 4);
-    k_cells<<<grid, block, 0, s>>>(mv, H, W, p, Dm, Sm, maxslot);
-    return hipGetLastError();
+    const int h = cells_halo(p);
+    const int hs = (h + 3) / 4 * 4;
+#define X(HS) \
+    if (hs <= HS) return launch_cells_hs<(HS > 0 ? HS : 4)>(mv, FH, FW, H, W, p, Dm, Sm, maxslot, s);
+    BIMP_CELL_HALOS(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s) {

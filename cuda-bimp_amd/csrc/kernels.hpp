@@ -3,11 +3,23 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace bimpk {
 
 constexpr int kNumOri = 8;
 constexpr int kUniqueOri = 5;  // orientations 5..7 mirror 3..1 in the spectrum
+
+// Scales of one resolution group (same pyramid level and transform size) processed by one
+// launch of each FFT stage.
+constexpr int kMaxBatch = 8;
+struct ScaleSet {
+    int n;
+    int kX[kMaxBatch];       // kernel centre = clamp-padding width
+    const float* G[kMaxBatch];
+    float2* DT[kMaxBatch];
+    float* C[kMaxBatch];
+};
 
 // Per-scale constants of the end-stop / inhibition stencil (host-derived).
 struct CellParams {
@@ -31,20 +43,24 @@ constexpr int kNmsPerThread = 16;
 constexpr int kNmsBlockPixels = kNmsThreads * kNmsPerThread;
 
 bool fft_size_supported(int n);
+// Packed per-stage twiddle table of the length-n transform (FP64-evaluated, FP32-stored).
+bool twiddle_table(int n, std::vector<float2>& out);
 // Register the dynamic-LDS limits of every kernel instance for size n (once per process).
 hipError_t prepare_size(int n);
+hipError_t prepare_common();
 
 hipError_t launch_init(unsigned* maxslots, int nslots, int* counter, hipStream_t s);
 hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, uint8_t* dst,
                            int dw, int dh, size_t dpitch, hipStream_t s);
-hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W, int kX,
-                           float scale, float2* R, int ldr, const float2* tw, hipStream_t s);
-hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, int H, int kY, int ncols,
-                           float2* DT, const float2* tw, hipStream_t s);
-hipError_t launch_inv_cols(int FH, const float2* DT, const float* G, int FW, int Hc, float2* T,
+hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W,
+                           const ScaleSet& ss, float scale, float2* R, int ldr, size_t rstride,
                            const float2* tw, hipStream_t s);
-hipError_t launch_inv_rows(int FW, const float2* T, int Hc, int Wc, float* C, int ldc,
-                           const float2* tw, hipStream_t s);
+hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, size_t rstride, int H,
+                           const ScaleSet& ss, int ncols, const float2* tw, hipStream_t s);
+hipError_t launch_inv_cols(int FH, const ScaleSet& ss, int FW, int Hc, float2* T,
+                           size_t tstride, const float2* tw, hipStream_t s);
+hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t tstride, int Hc,
+                           int Wc, int ldc, const float2* tw, hipStream_t s);
 hipError_t launch_cells(const float* C, int Hc, int ldc, int FH, int FW, int H, int W,
                         const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
                         hipStream_t s);

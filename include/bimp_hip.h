@@ -151,6 +151,8 @@ int bimp_set_map(bimp_ctx* ctx, int scale, int which, int orientation, const flo
 int bimp_keypoints_to_matrix(const bimp_keypoint* kps, int n, float* mat, int ld);
 int bimp_matrix_to_keypoints(const float* mat, int ld, int n, bimp_keypoint* kps);
 
+/* Message of the last failure on ctx; with ctx == NULL, of the last failed bimp_create on the
+ * calling thread. */
 const char* bimp_last_error(const bimp_ctx* ctx);
 const char* bimp_status_string(int status);
 
