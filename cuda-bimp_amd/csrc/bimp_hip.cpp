@@ -180,7 +180,21 @@ void free_all(bimp_ctx* c) {
     }
 }
 
-int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hipStream_t s) {
+int launch_cells_scale(bimp_ctx* c, size_t i, hipStream_t s) {
+    auto& sc = c->sc[i];
+    const auto& g = sc.g;
+    LaunchScope ls(c, s, F_CELLS, 4.0 * kNumOri * (g.H + 1.0) * (g.W + 1.0) + 8.0 * g.H * g.W);
+    HIPCHK(c, bimpk::launch_cells(sc.C, sc.Hc, sc.ldc, g.FH, g.FW, g.H, g.W, sc.cp, sc.D, sc.S,
+                                  c->maxslots + i, s));
+    return BIMP_OK;
+}
+
+// Pyramid, forward transforms (one launch per stage and resolution group), then per scale
+// the inverse column and row passes and, with `cells`, the end-stop/inhibition stencil.
+// Running a scale's consumers right after its producers keeps the intermediate T (one scale)
+// and the complex-cell maps resident in the 256 MB last-level cache.
+int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, bool cells,
+               hipStream_t s) {
     const int W0 = c->width, H0 = c->height;
     {
         LaunchScope ls(c, s, F_INIT, 0);
@@ -210,7 +224,6 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hip
     }
     for (const auto& gr : c->groups) {
         const auto& g = c->sc[gr.first].g;
-        const Scale& s0 = c->sc[gr.first];
         const int ncols = g.FW / 2 + 1;
         const float2* twW = c->tw[g.FW];
         const float2* twH = c->tw[g.FH];
@@ -224,7 +237,6 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hip
             ss.C[k] = sk.C;
         }
         const size_t rstride = (size_t)g.H * ncols;
-        const size_t tstride = (size_t)kNumOri * s0.Hc * g.FW;
         const double n = gr.n;
         {
             LaunchScope ls(c, s, F_FWD_ROWS, n * ((double)g.H * g.W + 8.0 * g.H * ncols), g.FW);
@@ -235,30 +247,42 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, hip
             LaunchScope ls(c, s, F_FWD_COLS, n * (8.0 * g.H * ncols + 8.0 * ncols * g.FH), g.FH);
             HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, rstride, g.H, ss, ncols, twH, s));
         }
-        {
-            LaunchScope ls(c, s, F_INV_COLS,
-                           n * (8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
-                                8.0 * kNumOri * s0.Hc * g.FW),
-                           g.FH);
-            HIPCHK(c, bimpk::launch_inv_cols(g.FH, ss, g.FW, s0.Hc, c->T, tstride, twH, s));
-        }
-        {
-            LaunchScope ls(c, s, F_INV_ROWS,
-                           n * (8.0 * kNumOri * s0.Hc * g.FW + 4.0 * kNumOri * s0.Hc * s0.Wc), g.FW);
-            HIPCHK(c, bimpk::launch_inv_rows(g.FW, ss, c->T, tstride, s0.Hc, s0.Wc, s0.ldc, twW, s));
+        for (int k = 0; k < gr.n; ++k) {
+            const Scale& sk = c->sc[gr.first + k];
+            bimpk::ScaleSet one;
+            one.n = 1;
+            one.kX[0] = sk.g.kX;
+            one.G[0] = sk.G;
+            one.DT[0] = sk.DT;
+            one.C[0] = sk.C;
+            const size_t tstride = (size_t)kNumOri * sk.Hc * g.FW;
+            {
+                LaunchScope ls(c, s, F_INV_COLS,
+                               8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
+                                   8.0 * kNumOri * sk.Hc * g.FW,
+                               g.FH);
+                HIPCHK(c, bimpk::launch_inv_cols(g.FH, one, g.FW, sk.Hc, c->T, tstride, twH, s));
+            }
+            {
+                LaunchScope ls(c, s, F_INV_ROWS,
+                               8.0 * kNumOri * sk.Hc * g.FW + 4.0 * kNumOri * sk.Hc * sk.Wc, g.FW);
+                HIPCHK(c, bimpk::launch_inv_rows(g.FW, one, c->T, tstride, sk.Hc, sk.Wc, sk.ldc,
+                                                 twW, s));
+            }
+            if (cells) {
+                const int st = launch_cells_scale(c, (size_t)(gr.first + k), s);
+                if (st) return st;
+            }
         }
     }
+    if (cells) c->maxValid = true;
     return BIMP_OK;
 }
 
 int run_cells(bimp_ctx* c, hipStream_t s) {
     for (size_t i = 0; i < c->sc.size(); ++i) {
-        auto& sc = c->sc[i];
-        const auto& g = sc.g;
-        LaunchScope ls(c, s, F_CELLS,
-                       4.0 * kNumOri * (g.H + 1.0) * (g.W + 1.0) + 8.0 * g.H * g.W);
-        HIPCHK(c, bimpk::launch_cells(sc.C, sc.Hc, sc.ldc, g.FH, g.FW, g.H, g.W, sc.cp, sc.D, sc.S,
-                                      c->maxslots + i, s));
+        const int st = launch_cells_scale(c, i, s);
+        if (st) return st;
     }
     c->maxValid = true;
     return BIMP_OK;
@@ -463,7 +487,7 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     for (const auto& gr : c->groups) {
         const Scale& s0 = c->sc[gr.first];
         rMax = std::max(rMax, (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
-        tMax = std::max(tMax, (size_t)gr.n * kNumOri * s0.Hc * s0.g.FW);
+        tMax = std::max(tMax, (size_t)kNumOri * s0.Hc * s0.g.FW);
     }
     for (auto& s : c->sc) {
         const auto& g = s.g;
@@ -566,7 +590,7 @@ int bimp_filter_u8(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device,
     if (!c || !img || pitch < (size_t)c->width) return fail(c, BIMP_E_ARG, "bad image arguments");
     hipStream_t s = (hipStream_t)stream;
     c->lastStream = s;
-    return run_filter(c, img, pitch, on_device, s);
+    return run_filter(c, img, pitch, on_device, false, s);
 }
 
 int bimp_cells(bimp_ctx* c, bimp_stream_t stream) {
@@ -590,9 +614,7 @@ int bimp_detect_u8(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_dev
         return fail(c, BIMP_E_ARG, "bad arguments");
     hipStream_t s = (hipStream_t)stream;
     c->lastStream = s;
-    int st = run_filter(c, img, pitch, img_on_device, s);
-    if (st) return st;
-    st = run_cells(c, s);
+    int st = run_filter(c, img, pitch, img_on_device, true, s);
     if (st) return st;
     return keypoints_out(c, out, cap, 0, out_on_device, count, s);
 }
@@ -604,9 +626,7 @@ int bimp_detect_u8_matrix(bimp_ctx* c, const uint8_t* img, size_t pitch, int img
         return fail(c, BIMP_E_ARG, "bad arguments");
     hipStream_t s = (hipStream_t)stream;
     c->lastStream = s;
-    int st = run_filter(c, img, pitch, img_on_device, s);
-    if (st) return st;
-    st = run_cells(c, s);
+    int st = run_filter(c, img, pitch, img_on_device, true, s);
     if (st) return st;
     return keypoints_out(c, mat, cap, ld, mat_on_device, count, s);
 }
@@ -784,8 +804,7 @@ int bimp_time_detect(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypo
     HIPCHK(c, hipEventCreate(&b));
     HIPCHK(c, hipEventRecord(a, s));
     for (int it = 0; it < iters; ++it) {
-        int st = run_filter(c, d_img, pitch, 1, s);
-        if (!st) st = run_cells(c, s);
+        int st = run_filter(c, d_img, pitch, 1, true, s);
         if (!st) st = run_keypoints(c, d_out, cap, 0, s);
         if (st) {
             (void)hipEventDestroy(a);

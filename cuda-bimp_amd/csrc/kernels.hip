@@ -269,7 +269,8 @@ __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __re
 #pragma unroll
         for (int e = 0; e < C::E; ++e) {
             const int x = out_index<FW>(e, lane);
-            if (x < Wc) out[x] = sqrtf(v[e].x * v[e].x + v[e].y * v[e].y);
+            // native v_sqrt_f32 (1 ulp): the map tolerance is 1e-4 relative
+            if (x < Wc) out[x] = __builtin_amdgcn_sqrtf(v[e].x * v[e].x + v[e].y * v[e].y);
         }
     }
 }

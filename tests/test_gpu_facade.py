@@ -1,0 +1,39 @@
+"""The C++ Context facade (include/bimp/bimp.hpp) on the GPU: a compiled caller gets the
+same keypoints as the Python binding of the same C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from kpmatch import bits
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_context_facade_matches_c_abi(tmp_path):
+    import bimp_amd as B
+    if B.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP library")
+    exe = tmp_path / "facade_main"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "cpp", "facade_main.cpp"),
+                           "-L", os.path.dirname(B.LIB_PATH), "-lbimp_hip",
+                           "-Wl,-rpath," + os.path.dirname(B.LIB_PATH), "-o", str(exe)])
+    w, h = 320, 240
+    img = O.synth_frame(w, h, 25, 31)
+    frame = tmp_path / "frame.u8"
+    img.tofile(frame)
+    out = tmp_path / "out.bin"
+    subprocess.check_call([str(exe), str(w), str(h), str(frame), str(out)], timeout=120)
+    raw = np.fromfile(out, np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    kps = raw[4:4 + 28 * n].view(B.KEYPOINT_DTYPE)
+    mat = raw[4 + 28 * n:].view(np.float32).reshape(7, n)
+    cs = B.CudaStuff(w, h, O.BASE_BANK)
+    want, m = cs.detect(img)
+    assert n == m
+    assert np.array_equal(bits(kps), bits(want))
+    assert np.array_equal(bits(B.matrix_to_keypoints(mat)), bits(want))
