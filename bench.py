@@ -6,6 +6,9 @@ filter bank, complex cells, end-stopping, inhibition, NMS and keypoint compactio
 device-resident frame to a device-resident keypoint list and count.
 
 Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel, CPU baseline).
+The same line carries the precision form (BF16 / FP16 direct convolution on the matrix cores
+for the scales with lambda <= 8, the others FFT-FP32): its frames/s, speed-up over the plain
+path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle.
 """
 import argparse
 import json
@@ -66,6 +69,90 @@ def cpu_baseline(img, frames):
                       f"(nproc {os.cpu_count()}), {n} keypoints/frame"}
 
 
+MFMA_PEAK_TFLOPS = 2500.0  # dense BF16 / FP16 matrix peak of MI355X
+
+
+def time_form(B, cs, d_img, d_out, cap, steps, warmup, stream):
+    if warmup:
+        cs.time_detect(d_img, W, d_out, cap, warmup, stream)
+    stream.sync()
+    ms = cs.time_detect(d_img, W, d_out, cap, steps, stream)
+    stream.sync()
+    return ms / steps
+
+
+def kernel_profile(B, cs, d_img, d_out, cap, frames, stream):
+    cs.set_flags(B.FLAG_PROFILE)
+    cs.reset_profile()
+    cs.time_detect(d_img, W, d_out, cap, frames, stream)
+    prof = cs.profile()
+    cs.set_flags(0)
+    return {k: v for k, v in prof.items() if k.startswith("k_")}
+
+
+def oracle_reference(img, direct_scales):
+    """FP64 oracle on the frame: keypoints of the reference filter, and the complex-cell maps
+    of the scales the precision form replaces (for its per-scale map error)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as O
+    O.lib().bo_set_threads(min(16, os.cpu_count() or 1))
+    orc = O.Oracle(W, H, bank(), fp64=True)
+    orc.set_image(img)
+    kps, maps = [], {}
+    for i in range(len(orc.scales)):
+        m = orc.filter(i)
+        if i in direct_scales:
+            maps[i] = m
+        D, _ = orc.cells(i, m)
+        k, _ = orc.nms(i, D)
+        kps.append(k)
+    return np.concatenate(kps), maps
+
+
+def precision_form(B, name, prec, nsig, plain_ms, ref_kps, ref_maps, d_img, d_out, cap,
+                   steps, warmup, stream, img, profile_frames):
+    from kpmatch import agreement, rel_err
+    opt = B.default_options()
+    opt.precision = prec
+    opt.direct_sigmas = nsig
+    cs = B.CudaStuff(W, H, bank(), opt)
+    ms = time_form(B, cs, d_img, d_out, cap, steps, warmup, stream)
+    kps, count = cs.detect(img)
+    errs = {}
+    for i, si in enumerate(cs.ks):
+        if si.precision != B.PRECISION_FP32 and i in ref_maps:
+            errs["%.4f" % si.lambda2] = float("%.3e" % max(
+                rel_err(cs.get_map(i, B.MAP_COMPLEX, o), ref_maps[i][o, :si.map_h, :si.map_w])
+                for o in range(8)))
+    kern = kernel_profile(B, cs, d_img, d_out, cap, profile_frames, stream)
+    # MFMA roofline of the direct convolution: 2 * 16 filters * ext^2 taps per map pixel
+    flops, dms = 0.0, 0.0
+    for si in cs.ks:
+        if si.precision != B.PRECISION_FP32:
+            flops += 2.0 * 16 * si.tap_extent ** 2 * si.map_h * si.map_w
+    for k, v in kern.items():
+        if k.startswith("k_direct"):
+            dms += v["total_ms"] / profile_frames
+    tflops = flops / (dms * 1e-3) / 1e12 if dms else None
+    return {
+        "fps": round(1000.0 / ms, 2), "ms_per_step": round(ms, 4),
+        "speedup_vs_plain": round(plain_ms / ms, 3),
+        "extent": "full kW" if nsig <= 0 else f"{nsig:g} sigma",
+        "direct_scales": ["%.4f" % si.lambda2 for si in cs.ks if si.precision != B.PRECISION_FP32],
+        "tap_extents": [si.tap_extent for si in cs.ks if si.precision != B.PRECISION_FP32],
+        "keypoints": int(count),
+        "agreement_vs_fp64_oracle": round(agreement(kps, ref_kps), 4),
+        "map_max_rel_err_vs_fp64": errs,
+        "mfma": {"bound": "mfma", "kernel": "k_direct", "achieved": round(tflops, 1) if tflops else None,
+                 "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(tflops / MFMA_PEAK_TFLOPS, 4) if tflops else None,
+                 "alg_flops_per_frame": flops, "ms_per_frame": round(dms, 4)},
+        "kernels_ms_per_frame": {k: round(v["total_ms"] / profile_frames, 4)
+                                 for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])},
+    }
+
+
 def load_traffic(kernel):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -83,6 +170,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-frames", type=int, default=20)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the BF16/FP16 precision-form lines and the FP64 oracle check")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -159,6 +248,24 @@ def main():
         "kernel_sum_ms_per_frame": round(frame_kernel_ms, 4),
         "wall_s_timed": round(wall, 3),
     }
+    if not args.no_precision:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from kpmatch import agreement
+        direct = [i for i, si in enumerate(cs.ks) if si.lambda2 <= 8.0]
+        t0 = time.perf_counter()
+        ref_kps, ref_maps = oracle_reference(img, direct)
+        print(f"fp64 oracle on the frame: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+        out["keypoint_agreement_vs_fp64_oracle"] = round(agreement(kps, ref_kps), 4)
+        forms = {}
+        for name, prec, nsig in (("bf16", B.PRECISION_BF16, 0.0), ("fp16", B.PRECISION_FP16, 0.0),
+                                 ("bf16_6sigma", B.PRECISION_BF16, 6.0)):
+            forms[name] = precision_form(B, name, prec, nsig, ms_per_step, ref_kps, ref_maps,
+                                         d_img, d_out, cap, args.steps, args.warmup, stream, img,
+                                         args.profile_frames)
+            print(f"precision form {name}: {forms[name]['fps']} frames/s", file=sys.stderr,
+                  flush=True)
+        out["precision_form"] = forms
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_frames)
     else:

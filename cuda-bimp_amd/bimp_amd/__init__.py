@@ -23,6 +23,8 @@ KP_MAX_POINTS = 8000
 OK, W_OVERFLOW, E_ARG, E_ALLOC, E_DEVICE, E_UNSUPPORTED = 0, 1, -1, -2, -3, -4
 MAP_DOUBLE, MAP_SINGLE, MAP_COMPLEX = 0, 1, 2
 FLAG_PROFILE = 1
+PRECISION_FP32, PRECISION_BF16, PRECISION_FP16 = 0, 1, 2
+DIRECT_MAX_EXTENT = 63
 
 # SURF_CUDA keypoint-matrix rows (utils.cpp:109-150)
 X_ROW, Y_ROW, LAPLACIAN_ROW, OCTAVE_ROW, SIZE_ROW, ANGLE_ROW, HESSIAN_ROW = range(7)
@@ -40,7 +42,8 @@ class BimpError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("gamma", C.c_float), ("sigma_ratio", C.c_float), ("support", C.c_float),
                 ("pyr_threshold", C.c_float), ("nms_threshold", C.c_double),
-                ("input_scale", C.c_float), ("flags", C.c_int)]
+                ("input_scale", C.c_float), ("flags", C.c_int), ("precision", C.c_int),
+                ("direct_max_lambda", C.c_float), ("direct_sigmas", C.c_float)]
 
 
 class ScaleInfo(C.Structure):
@@ -48,7 +51,8 @@ class ScaleInfo(C.Structure):
                 ("kernel_size", C.c_int), ("kernel_centre", C.c_int), ("border", C.c_int),
                 ("level", C.c_int), ("data_h", C.c_int), ("data_w", C.c_int),
                 ("fft_h", C.c_int), ("fft_w", C.c_int), ("map_h", C.c_int),
-                ("map_w", C.c_int), ("map_pitch", C.c_int)]
+                ("map_w", C.c_int), ("map_pitch", C.c_int), ("precision", C.c_int),
+                ("tap_extent", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

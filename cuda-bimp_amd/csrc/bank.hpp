@@ -6,6 +6,7 @@
 // FP64 before a single rounding to FP32.
 #pragma once
 #include <complex>
+#include <cstdint>
 #include <vector>
 
 namespace bimph {
@@ -21,11 +22,38 @@ int snap_transform_size(int n);
 bool derive_scale(float lambda2, int width, int height, float pyr_threshold, float support,
                   ScaleGeom* g);
 // Even/odd taps of orientation o (cuda_bimp.cu:304-331), kW*kW each, row-major.
-void gabor_taps(const ScaleGeom& g, int o, float gamma, float sigma_ratio, float* ke, float* ko);
+// dc_out, when given, receives the subtracted DC offset evensum / fs^2.
+void gabor_taps(const ScaleGeom& g, int o, float gamma, float sigma_ratio, float* ke, float* ko,
+                float* dc_out = nullptr);
 // Quadrature spectra G_o(ky, kx) = Re[DFT2(Ke + i*Ko)] / (FH*FW) for o = 0..4, stored
 // column-major: out[(o*FW + kx)*FH + ky].  DFT(Ke) is real and DFT(Ko) imaginary (Ke even,
 // Ko odd), so the real part carries the whole quadrature pair.
 void quadrature_bank(const ScaleGeom& g, float gamma, float sigma_ratio, int threads,
                      std::vector<float>& out);
+
+// Precision form: the 16 filters of one scale (even and odd kernel of each orientation,
+// o = 0..7, filter index 2*o and 2*o+1) flipped for convolution, scaled by 1/fscale and
+// rounded once to BF16 or FP16, laid out as the B operand of v_mfma_f32_16x16x32:
+//   frag[(ks*64 + lane)*8 + t] = B[32*ks + 8*(lane/16) + t][lane%16]
+// with K index k = u*extp + v (u, v < ext; zero elsewhere).  nsig <= 0 (or an extent that
+// reaches kW) keeps the whole kW x kW kernel Ke; nsig > 0 truncates to the odd extent
+// ceil(nsig*sigma) and the even filter becomes the carrier Ke + dc inside the window.  The
+// caller subtracts dc[f] times the kW x kW box sum of the image from filter f's response,
+//   dc[f] = (fscale * sum(rounded taps of f) - sum(exact kW x kW kernel of f)) / kW^2,
+// so that a constant image gets the exact full kernel's response.
+struct DirectBank {
+    int ext = 0, R = 0, extp = 0, ksteps = 0;
+    float fscale = 1.f;
+    float dc[16] = {0};
+    std::vector<uint16_t> frag;
+};
+void direct_bank(const ScaleGeom& g, float gamma, float sigma_ratio, float nsig, bool fp16,
+                 DirectBank& out);
+// Extent direct_bank would choose (without building the taps).
+int direct_extent(const ScaleGeom& g, float sigma_ratio, float nsig);
+uint16_t float_to_bf16(float f);  // round to nearest even
+uint16_t float_to_f16(float f);   // round to nearest even
+double bf16_to_double(uint16_t h);
+double f16_to_double(uint16_t h);
 
 }  // namespace bimph

@@ -26,12 +26,13 @@ namespace {
 
 enum Family {
     F_INIT, F_COPY_IN, F_PYR, F_FWD_ROWS, F_FWD_COLS, F_INV_COLS, F_INV_ROWS, F_CELLS,
-    F_MAX, F_NMS_COUNT, F_NMS_SCAN, F_NMS_WRITE, F_COUNT
+    F_MAX, F_NMS_COUNT, F_NMS_SCAN, F_NMS_WRITE, F_BOX, F_DIRECT, F_COUNT
 };
 // Profile names follow the kernel symbols (size-templated kernels carry their length).
 const char* kFamilyNames[F_COUNT] = {"k_init",      "copy_in",    "k_pyr_down",  "k_fwd_rows",
                                      "k_fwd_cols",  "k_inv_cols", "k_inv_rows",  "k_cells",
-                                     "k_max",       "k_nms_count", "k_nms_scan", "k_nms_write"};
+                                     "k_max",       "k_nms_count", "k_nms_scan", "k_nms_write",
+                                     "k_box",       "k_direct"};
 
 struct Scale {
     bimph::ScaleGeom g;
@@ -42,6 +43,16 @@ struct Scale {
     float* C = nullptr;    // 8 x Hc x ldc
     float* D = nullptr;    // H x W
     float* S = nullptr;    // H x W
+    // direct (low-precision) form
+    int precision = BIMP_PRECISION_FP32;
+    bimpk::DirectScale ds{};
+    void* frag = nullptr;  // B-operand fragments, ksteps x 64 lanes x 8 taps (16-bit)
+};
+
+// One step of a frame: an FFT resolution group or one direct-form scale.
+struct Step {
+    bool direct;
+    int index;  // group index or scale index
 };
 
 // Scales sharing pyramid level and transform size; each FFT stage runs once per group.
@@ -67,11 +78,14 @@ struct bimp_ctx {
     bimp_options opt;
     std::vector<Scale> sc;
     std::vector<Group> groups;
+    std::vector<Step> steps;
     int maxLevel = 0;
     std::vector<uint8_t*> level;  // level[0] is the staging copy of a host frame
     std::vector<int> levelW, levelH;
     float2* R = nullptr;
     float2* T = nullptr;
+    int* boxRows = nullptr;  // direct form with truncated taps: horizontal box sums
+    float* box = nullptr;    // and the kW x kW box sums
     std::map<int, float2*> tw;
     unsigned* maxslots = nullptr;
     int* counter = nullptr;
@@ -164,7 +178,10 @@ void free_all(bimp_ctx* c) {
         (void)hipFree(s.C);
         (void)hipFree(s.D);
         (void)hipFree(s.S);
+        (void)hipFree(s.frag);
     }
+    (void)hipFree(c->boxRows);
+    (void)hipFree(c->box);
     for (auto* p : c->level) (void)hipFree(p);
     for (auto& kv : c->tw) (void)hipFree(kv.second);
     (void)hipFree(c->R);
@@ -191,6 +208,8 @@ int launch_cells_scale(bimp_ctx* c, size_t i, hipStream_t s) {
 
 // Pyramid, forward transforms (one launch per stage and resolution group), then per scale
 // the inverse column and row passes and, with `cells`, the end-stop/inhibition stencil.
+// Direct-form scales replace the transforms by one MFMA convolution launch (plus the box sums
+// when their taps are truncated).
 // Running a scale's consumers right after its producers keeps the intermediate T (one scale)
 // and the complex-cell maps resident in the 256 MB last-level cache.
 int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, bool cells,
@@ -222,7 +241,31 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         lv[l] = c->level[l];
         lp[l] = (size_t)c->levelW[l];
     }
-    for (const auto& gr : c->groups) {
+    for (const Step& st : c->steps) {
+        if (st.direct) {
+            const int i = st.index;
+            const Scale& sk = c->sc[i];
+            const auto& g = sk.g;
+            {
+                LaunchScope ls(c, s, F_BOX,
+                               (double)g.H * g.W + 4.0 * sk.Hc * sk.Wc);
+                HIPCHK(c, bimpk::launch_box(lv[g.level], lp[g.level], sk.ds, g.kX, c->boxRows,
+                                            c->box, s));
+            }
+            {
+                LaunchScope ls(c, s, F_DIRECT,
+                               (double)g.H * g.W + 4.0 * kNumOri * sk.Hc * sk.Wc, sk.ds.ext);
+                HIPCHK(c, bimpk::launch_direct(lv[g.level], lp[g.level], sk.ds,
+                                               sk.precision == BIMP_PRECISION_FP16, sk.frag,
+                                               c->box, sk.C, s));
+            }
+            if (cells) {
+                const int r = launch_cells_scale(c, (size_t)i, s);
+                if (r) return r;
+            }
+            continue;
+        }
+        const Group& gr = c->groups[st.index];
         const auto& g = c->sc[gr.first].g;
         const int ncols = g.FW / 2 + 1;
         const float2* twW = c->tw[g.FW];
@@ -381,6 +424,9 @@ void bimp_default_options(bimp_options* o) {
     o->nms_threshold = 0.104;
     o->input_scale = (float)(255.0 / 65536.0);
     o->flags = 0;
+    o->precision = BIMP_PRECISION_FP32;
+    o->direct_max_lambda = 8.0f;
+    o->direct_sigmas = 0.f;
 }
 
 int bimp_snap_transform_size(int n) { return n > 0 ? bimph::snap_transform_size(n) : BIMP_E_ARG; }
@@ -411,6 +457,12 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         c->opt = *opt;
     else
         bimp_default_options(&c->opt);
+    if (c->opt.precision < BIMP_PRECISION_FP32 || c->opt.precision > BIMP_PRECISION_FP16 ||
+        !(c->opt.direct_sigmas >= 0.f)) {
+        delete c;
+        g_create_error = "invalid precision options";
+        return BIMP_E_ARG;
+    }
     std::vector<float> lam(lambdas, lambdas + n);
     std::sort(lam.begin(), lam.end());  // cuda_bimp.cu:224
     for (float l : lam) {
@@ -421,8 +473,12 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             g_create_error = "wavelength " + std::to_string(l) + " gives an invalid scale";
             return BIMP_E_ARG;
         }
-        if (!bimpk::fft_size_supported(s.g.FH) || !bimpk::fft_size_supported(s.g.FW) ||
-            s.g.FW % bimpk::col_batch(s.g.FH) != 0) {
+        if (c->opt.precision != BIMP_PRECISION_FP32 && l <= c->opt.direct_max_lambda &&
+            bimph::direct_extent(s.g, c->opt.sigma_ratio, c->opt.direct_sigmas) <= BIMP_DIRECT_MAX_EXTENT)
+            s.precision = c->opt.precision;
+        if (s.precision == BIMP_PRECISION_FP32 &&
+            (!bimpk::fft_size_supported(s.g.FH) || !bimpk::fft_size_supported(s.g.FW) ||
+             s.g.FW % bimpk::col_batch(s.g.FH) != 0)) {
             delete c;
             g_create_error = "transform size " + std::to_string(s.g.FH) + " x " +
                              std::to_string(s.g.FW) + " is not supported";
@@ -466,10 +522,17 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     for (int l = 0; l <= c->maxLevel; ++l)
         if (dalloc(&c->level[l], (size_t)c->levelW[l] * c->levelH[l]) != hipSuccess)
             return bail(BIMP_E_ALLOC, "level");
-    // resolution groups: consecutive scales (ascending lambda) with one level and size
+    // resolution groups: consecutive FFT-form scales (ascending lambda) with one level and
+    // size; a direct-form scale is a step of its own
+    bool prevFft = false;
     for (int i = 0; i < (int)c->sc.size(); ++i) {
         const auto& g = c->sc[i].g;
-        if (!c->groups.empty()) {
+        if (c->sc[i].precision != BIMP_PRECISION_FP32) {
+            c->steps.push_back({true, i});
+            prevFft = false;
+            continue;
+        }
+        if (prevFft) {
             Group& last = c->groups.back();
             const auto& h0 = c->sc[last.first].g;
             if (h0.level == g.level && h0.H == g.H && h0.W == g.W && h0.FH == g.FH &&
@@ -479,6 +542,8 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             }
         }
         c->groups.push_back({i, 1});
+        c->steps.push_back({false, (int)c->groups.size() - 1});
+        prevFft = true;
     }
     size_t rMax = 0, tMax = 0;
     std::vector<int> sizes;
@@ -489,28 +554,70 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         rMax = std::max(rMax, (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
         tMax = std::max(tMax, (size_t)kNumOri * s0.Hc * s0.g.FW);
     }
+    size_t boxRowsMax = 0, boxMax = 0;
+    bool anyDirect = false;
     for (auto& s : c->sc) {
         const auto& g = s.g;
-        const int ncols = g.FW / 2 + 1;
-        sizes.push_back(g.FH);
-        sizes.push_back(g.FW);
-        if (dalloc(&s.G, (size_t)kUniqueOri * g.FW * g.FH) != hipSuccess ||
-            dalloc(&s.DT, (size_t)ncols * g.FH) != hipSuccess ||
-            dalloc(&s.C, (size_t)kNumOri * s.Hc * s.ldc) != hipSuccess ||
+        if (dalloc(&s.C, (size_t)kNumOri * s.Hc * s.ldc) != hipSuccess ||
             dalloc(&s.D, (size_t)g.H * g.W) != hipSuccess ||
             dalloc(&s.S, (size_t)g.H * g.W) != hipSuccess)
             return bail(BIMP_E_ALLOC, "scale buffers");
-        std::vector<float> bank;
-        bimph::quadrature_bank(g, c->opt.gamma, c->opt.sigma_ratio, threads, bank);
-        if (hipMemcpy(s.G, bank.data(), bank.size() * sizeof(float), hipMemcpyHostToDevice) !=
-            hipSuccess)
-            return bail(BIMP_E_DEVICE, "bank upload");
+        if (s.precision != BIMP_PRECISION_FP32) {
+            anyDirect = true;
+            bimph::DirectBank db;
+            bimph::direct_bank(g, c->opt.gamma, c->opt.sigma_ratio, c->opt.direct_sigmas,
+                               s.precision == BIMP_PRECISION_FP16, db);
+            auto& d = s.ds;
+            d.H = g.H;
+            d.W = g.W;
+            d.FH = g.FH;
+            d.FW = g.FW;
+            d.kpad = g.kX;
+            d.Hc = s.Hc;
+            d.Wc = s.Wc;
+            d.ldc = s.ldc;
+            d.R = db.R;
+            d.ext = db.ext;
+            d.extp = db.extp;
+            d.ksteps = db.ksteps;
+            d.lwp = bimpk::direct_lwp(db.R, db.extp);
+            d.fscale = db.fscale;
+            d.iscale = c->opt.input_scale;
+            for (int f = 0; f < 2 * kNumOri; ++f) d.dc[f] = db.dc[f];
+            if (bimpk::direct_lds_bytes(d) > bimpk::kDirectMaxLds)
+                return bail(BIMP_E_UNSUPPORTED, "direct-form tile exceeds LDS");
+            boxRowsMax = std::max(boxRowsMax, (size_t)(s.Hc + 2 * g.kX) * s.Wc);
+            boxMax = std::max(boxMax, (size_t)s.Hc * s.Wc);
+            if (dalloc((uint16_t**)&s.frag, db.frag.size()) != hipSuccess)
+                return bail(BIMP_E_ALLOC, "direct taps");
+            if (hipMemcpy(s.frag, db.frag.data(), db.frag.size() * sizeof(uint16_t),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return bail(BIMP_E_DEVICE, "direct taps upload");
+        } else {
+            const int ncols = g.FW / 2 + 1;
+            sizes.push_back(g.FH);
+            sizes.push_back(g.FW);
+            if (dalloc(&s.G, (size_t)kUniqueOri * g.FW * g.FH) != hipSuccess ||
+                dalloc(&s.DT, (size_t)ncols * g.FH) != hipSuccess)
+                return bail(BIMP_E_ALLOC, "scale buffers");
+            std::vector<float> bank;
+            bimph::quadrature_bank(g, c->opt.gamma, c->opt.sigma_ratio, threads, bank);
+            if (hipMemcpy(s.G, bank.data(), bank.size() * sizeof(float), hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return bail(BIMP_E_DEVICE, "bank upload");
+        }
         // complex-cell maps start at zero so the unused padding columns are defined
         if (hipMemset(s.C, 0, (size_t)kNumOri * s.Hc * s.ldc * sizeof(float)) != hipSuccess)
             return bail(BIMP_E_DEVICE, "memset");
     }
-    if (dalloc(&c->R, rMax) != hipSuccess || dalloc(&c->T, tMax) != hipSuccess)
+    if ((rMax && dalloc(&c->R, rMax) != hipSuccess) || (tMax && dalloc(&c->T, tMax) != hipSuccess))
         return bail(BIMP_E_ALLOC, "scratch");
+    if (boxMax && (dalloc(&c->boxRows, boxRowsMax) != hipSuccess || dalloc(&c->box, boxMax) != hipSuccess))
+        return bail(BIMP_E_ALLOC, "box sums");
+    if (anyDirect) {
+        if (hipError_t e = bimpk::prepare_direct(); e != hipSuccess)
+            return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));
+    }
     if (hipError_t e = bimpk::prepare_common(); e != hipSuccess)
         return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));
     std::sort(sizes.begin(), sizes.end());
@@ -583,6 +690,8 @@ int bimp_get_scale_info(const bimp_ctx* c, int i, bimp_scale_info* o) {
     o->map_h = s.Hc;
     o->map_w = s.Wc;
     o->map_pitch = s.ldc;
+    o->precision = s.precision;
+    o->tap_extent = s.precision != BIMP_PRECISION_FP32 ? s.ds.ext : s.g.kW;
     return BIMP_OK;
 }
 

@@ -284,15 +284,6 @@ __device__ __forceinline__ float key_float(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-// Inclusive validity of cuda_bimp.cuh:446-449 / :479-482.
-__device__ __forceinline__ bool stencil_valid(int x, int y, int o1, int o2, int W, int H) {
-    if (x - o2 < 0 || x - o2 > W || x + o2 < 0 || x + o2 > W) return false;
-    if (y - o1 < 0 || y - o1 > H || y + o1 < 0 || y + o1 > H) return false;
-    if (x - 2 * o2 < 0 || x - 2 * o2 > W || x + 2 * o2 < 0 || x + 2 * o2 > W) return false;
-    if (y - 2 * o1 < 0 || y - 2 * o1 > H || y + 2 * o1 < 0 || y + 2 * o1 > H) return false;
-    return true;
-}
-
 // The reference addresses complex maps linearly (y*FW + x over an FH x FW array).  Row y <= H
 // and column x <= W: x == FW continues on the next row, and a row index of FH wraps to row 0
 // (one element past the reference's array, undefined there).

@@ -38,6 +38,25 @@ struct NmsScale {
     int slot;           // index of the scale's max slot
 };
 
+// One scale of the precision form (direct MFMA convolution, see direct.hip).
+struct DirectScale {
+    int H, W, FH, FW, kpad;  // data size, transform size, clamp-padding width (full kernel kX)
+    int Hc, Wc, ldc;         // complex-cell map region and pitch
+    int R, ext, extp;        // truncated half-extent, extent (odd), row length padded to 8
+    int ksteps, lwp;         // K steps of 32 taps; LDS row pitch of the staged tile
+    float fscale, iscale;    // filter scale, input scale (255/65536)
+    float dc[2 * kNumOri];   // box coefficients, filter 2o even, 2o+1 odd
+};
+
+constexpr size_t kDirectMaxLds = 150 * 1024;
+size_t direct_lds_bytes(const DirectScale& ds);
+int direct_lwp(int R, int extp);
+hipError_t prepare_direct();
+hipError_t launch_box(const uint8_t* img, size_t pitch, const DirectScale& ds, int kfull,
+                      int* hs, float* box, hipStream_t s);
+hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectScale& ds, bool fp16,
+                         const void* bfrag, const float* box, float* C, hipStream_t s);
+
 constexpr int kNmsThreads = 256;
 constexpr int kNmsPerThread = 16;
 constexpr int kNmsBlockPixels = kNmsThreads * kNmsPerThread;

@@ -67,9 +67,27 @@ typedef struct bimp_options {
                              reference multiplies in double, cuda_bimp.cu:721) */
     float input_scale;    /* 255/65536: uint8 -> float conversion factor */
     int flags;            /* BIMP_FLAG_* */
+    int precision;        /* BIMP_PRECISION_*: arithmetic of the Gabor bank (the reference
+                             runs FP32 cuFFT, cuda_bimp.cu:486-507) */
+    float direct_max_lambda; /* low-precision forms: scales whose requested wavelength is
+                                at most this (full-resolution pixels) run the direct form */
+    float direct_sigmas;  /* low-precision forms: 0 convolves with the full kW x kW kernel;
+                             s > 0 truncates the taps to an odd extent of ceil(s*sigma).
+                             Either way an exact integer box sum restores every filter's
+                             response to a constant image */
 } bimp_options;
 
 #define BIMP_FLAG_PROFILE 1  /* time every kernel with HIP events (see bimp_get_profile) */
+
+/* Arithmetic of the Gabor bank.  FP32: FFT convolution in FP32 (the reference's form).
+ * BF16 / FP16: direct spatial convolution on the matrix cores, uint8 samples times taps
+ * rounded once to BF16 / FP16, accumulated in FP32; used for every scale with
+ * lambda <= direct_max_lambda whose kernel extent is at most BIMP_DIRECT_MAX_EXTENT, the
+ * others keep the FP32 FFT form. */
+#define BIMP_PRECISION_FP32 0
+#define BIMP_PRECISION_BF16 1
+#define BIMP_PRECISION_FP16 2
+#define BIMP_DIRECT_MAX_EXTENT 63
 
 typedef struct bimp_scale_info {
     float lambda2;   /* requested wavelength (full resolution) */
@@ -82,6 +100,8 @@ typedef struct bimp_scale_info {
     int data_h, data_w;
     int fft_h, fft_w;
     int map_h, map_w, map_pitch; /* stored complex-cell region and its row pitch (floats) */
+    int precision;               /* BIMP_PRECISION_* this scale runs in */
+    int tap_extent;              /* direct forms: convolved kernel extent (odd), else kW */
 } bimp_scale_info;
 
 typedef struct bimp_ctx bimp_ctx;

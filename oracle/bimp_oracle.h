@@ -89,6 +89,24 @@ int bo_nms_scale(const bo_ctx* c, int i, const float* D, bo_keypoint* out, int c
 int bo_detect(bo_ctx* c, const uint8_t* img, size_t pitch, bo_keypoint* out, int cap,
               int* count, float* D_out, float* S_out);
 
+/* Precision form of the library (no counterpart in the reference, which filters in FP32 via
+ * cuFFT).  Taps: the even kernel Ke (nsig <= 0: the whole kW x kW kernel) or, truncated to
+ * the centred odd extent ext = ceil(nsig * 0.56 * lambda) < kW, the carrier Ke + dc inside
+ * the window; the odd kernel Ko.  fscale = max |tap| over the 16 filters (as float); each
+ * tap is stored as round((float)(tap / fscale)) to BF16 (fp16 = 0) or FP16, nearest-even.
+ * taps receives the rounded values as 16 kW x kW kernels (filter 2o even, 2o+1 odd; zero
+ * outside the extent).  dc receives 16 box coefficients, one per filter f,
+ *   dc_f = (fscale * sum(rounded taps of f) - sum(exact kW x kW kernel of f)) / kW^2
+ * which make the response to a constant image that of the exact full kernel. */
+int bo_direct_taps(const bo_scale* s, float nsig, int fp16, int* ext, double* fscale,
+                   double* dc, double* taps);
+/* Complex-cell maps of scale i in the precision form, computed in FP64:
+ *   e_o = fscale * sum(tap * u8 sample) - dc_2o * (kW x kW box sum),
+ *   o_o = fscale * sum(tap * u8 sample) - dc_2o+1 * (kW x kW box sum),
+ *   C_o = (255/65536) * sqrt(e_o^2 + o_o^2)
+ * over the periodic padded image, 8 maps of fftH x fftW as bo_filter_scale. */
+int bo_direct_filter_scale(bo_ctx* c, int i, float nsig, int fp16, float* maps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,9 @@ def lib():
         L.bo_cells_scale.argtypes = [P, C.c_int, P, P, P]
         L.bo_nms_scale.argtypes = [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_int)]
         L.bo_detect.argtypes = [P, P, C.c_size_t, P, C.c_int, C.POINTER(C.c_int), P, P]
+        L.bo_direct_taps.argtypes = [C.POINTER(Scale), C.c_float, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), P, P]
+        L.bo_direct_filter_scale.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
         L.bimp_synth_frame.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, P, C.c_size_t]
         _lib = L
     return _lib
@@ -101,6 +104,17 @@ def gabor(s, o):
     ko = np.zeros((s.kW, s.kW), np.float32)
     lib().bo_gabor(C.byref(s), o, ptr(ke), ptr(ko))
     return ke, ko
+
+
+def direct_taps(s, nsig=0.0, fp16=False):
+    """(ext, fscale, dc[16], taps[16, kW, kW]) of the precision form."""
+    ext = C.c_int()
+    fscale = C.c_double()
+    dc = np.zeros(16, np.float64)
+    taps = np.zeros((16, s.kW, s.kW), np.float64)
+    assert lib().bo_direct_taps(C.byref(s), nsig, int(fp16), C.byref(ext), C.byref(fscale),
+                                ptr(dc), ptr(taps)) == 0
+    return ext.value, fscale.value, dc, taps
 
 
 def pyr_down(img):
@@ -180,6 +194,13 @@ class Oracle:
         s = self.scales[i]
         maps = np.zeros((8, s.fftH, s.fftW), np.float32)
         assert lib().bo_filter_scale(self.c, i, ptr(maps)) == 0
+        return maps
+
+    def direct_filter(self, i, nsig=0.0, fp16=False):
+        """Precision-form complex-cell maps (FP64 with rounded taps), 8 x fftH x fftW."""
+        s = self.scales[i]
+        maps = np.zeros((8, s.fftH, s.fftW), np.float32)
+        assert lib().bo_direct_filter_scale(self.c, i, nsig, int(fp16), ptr(maps)) == 0
         return maps
 
     def cells(self, i, maps):
