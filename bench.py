@@ -8,7 +8,9 @@ device-resident frame to a device-resident keypoint list and count.
 Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel, CPU baseline).
 The same line carries the precision form (BF16 / FP16 direct convolution on the matrix cores
 for the scales with lambda <= 8, the others FFT-FP32): its frames/s, speed-up over the plain
-path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle.
+path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle,
+and the batching form (64 frames of five sizes as one tuned hipGraph launch): aggregate
+megapixels/s and per-frame latency.
 """
 import argparse
 import json
@@ -118,7 +120,8 @@ def precision_form(B, name, prec, nsig, plain_ms, ref_kps, ref_maps, d_img, d_ou
     opt.direct_sigmas = nsig
     cs = B.CudaStuff(W, H, bank(), opt)
     ms = time_form(B, cs, d_img, d_out, cap, steps, warmup, stream)
-    kps, count = cs.detect(img)
+    # every keypoint of the frame (the reference's 8000 cap would truncate the comparison)
+    kps, count = cs.detect(img, cap=1 << 16)
     errs = {}
     for i, si in enumerate(cs.ks):
         if si.precision != B.PRECISION_FP32 and i in ref_maps:
@@ -153,6 +156,42 @@ def precision_form(B, name, prec, nsig, plain_ms, ref_kps, ref_maps, d_img, d_ou
     }
 
 
+BATCH_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def batch_form(B, stream, frames=64, seed=3, iters=3):
+    """64 frames with sizes drawn uniformly from BATCH_SIZES (seeded), 40 shapes each, frame i
+    from seed 1000 + i; one tuned hipGraph launch per batch."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [BATCH_SIZES[k] for k in rng.integers(0, len(BATCH_SIZES), frames)]
+    t0 = time.perf_counter()
+    bt = B.Batch(sizes, bank(), tune=True)
+    setup = time.perf_counter() - t0
+    for i, (w, h) in enumerate(sizes):
+        bt.upload(i, B.synth_frame(w, h, 40, 1000 + i))
+    ms, per = bt.time(iters, stream)
+    mpix = sum(w * h for w, h in sizes) / 1e6
+    counts = {}
+    for w, h in sizes:
+        counts[f"{w}x{h}"] = counts.get(f"{w}x{h}", 0) + 1
+    out = {
+        "frames": frames, "sizes": counts, "megapixels_per_batch": round(mpix, 3),
+        "batch_ms": round(ms, 3), "mpix_per_s": round(mpix / (ms * 1e-3), 1),
+        "frames_per_s": round(frames / (ms * 1e-3), 1),
+        "latency_alone_ms": {"p50": round(float(np.percentile(per, 50)), 4),
+                             "p99": round(float(np.percentile(per, 99)), 4),
+                             "by_size": {k: round(float(np.median([per[i] for i in range(frames)
+                                                                  if f"{sizes[i][0]}x{sizes[i][1]}" == k])), 4)
+                                         for k in counts}},
+        "serial_sum_ms": round(float(per.sum()), 3),
+        "overlap_gain": round(float(per.sum()) / ms, 3),
+        "plan": bt.plan(), "setup_s": round(setup, 1),
+    }
+    bt.close()
+    return out
+
+
 def load_traffic(kernel):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -172,6 +211,12 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=20)
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the BF16/FP16 precision-form lines and the FP64 oracle check")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form line")
+    ap.add_argument("--form", choices=["plain", "bf16", "fp16"], default="plain",
+                    help="path timed for the main line (plain = FFT-FP32, the headline)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, launch the library's plain copy kernel once over MIB "
+                         "MiB (the reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -185,7 +230,10 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     img = B.synth_frame(W, H, NSHAPES, SEED)
-    cs = B.CudaStuff(W, H, bank())
+    opt = B.default_options()
+    if args.form != "plain":
+        opt.precision = B.PRECISION_BF16 if args.form == "bf16" else B.PRECISION_FP16
+    cs = B.CudaStuff(W, H, bank(), opt)
     stream = B.Stream()
     d_img = B.DeviceBuffer(W * H)
     d_img.upload(img)
@@ -202,7 +250,8 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = ms / args.steps
     fps = 1000.0 / ms_per_step
-    kps, count = cs.detect(img)
+    # every keypoint of the frame (the reference's 8000 cap would truncate the comparison)
+    kps, count = cs.detect(img, cap=1 << 16)
 
     # per-kernel profile (HIP events around every launch), separate from the timed region
     cs.set_flags(B.FLAG_PROFILE)
@@ -230,9 +279,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.form == "plain" else args.form,
         "data": "synthetic",
-        "config": {"workload": "BIMP plain FFT-FP32, one 1920x1080 uint8 frame, 300 shapes, "
+        "config": {"workload": ("BIMP plain FFT-FP32" if args.form == "plain" else
+                                f"BIMP precision form ({args.form} direct for lambda <= 8)") +
+                               ", one 1920x1080 uint8 frame, 300 shapes, "
                                "seed 1, 8 wavelengths 4*2^(k/4) x 8 orientations",
                    "frame": [W, H], "scales": len(cs.ks), "orientations": 8,
                    "keypoints": int(count)},
@@ -248,7 +299,7 @@ def main():
         "kernel_sum_ms_per_frame": round(frame_kernel_ms, 4),
         "wall_s_timed": round(wall, 3),
     }
-    if not args.no_precision:
+    if not args.no_precision and args.form == "plain":
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from kpmatch import agreement
         direct = [i for i, si in enumerate(cs.ks) if si.lambda2 <= 8.0]
@@ -259,6 +310,8 @@ def main():
         out["keypoint_agreement_vs_fp64_oracle"] = round(agreement(kps, ref_kps), 4)
         forms = {}
         for name, prec, nsig in (("bf16", B.PRECISION_BF16, 0.0), ("fp16", B.PRECISION_FP16, 0.0),
+                                 ("bf16_8sigma", B.PRECISION_BF16, 8.0),
+                                 ("fp16_8sigma", B.PRECISION_FP16, 8.0),
                                  ("bf16_6sigma", B.PRECISION_BF16, 6.0)):
             forms[name] = precision_form(B, name, prec, nsig, ms_per_step, ref_kps, ref_maps,
                                          d_img, d_out, cap, args.steps, args.warmup, stream, img,
@@ -266,11 +319,19 @@ def main():
             print(f"precision form {name}: {forms[name]['fps']} frames/s", file=sys.stderr,
                   flush=True)
         out["precision_form"] = forms
+    if not args.no_batch and args.form == "plain":
+        out["batch_form"] = batch_form(B, stream)
+        print(f"batch form: {out['batch_form']['mpix_per_s']} MPix/s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_frames)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        a, b = B.DeviceBuffer(nb), B.DeviceBuffer(nb)
+        B.lib.bimp_copy_kernel(b.ptr, a.ptr, nb, stream.handle)
+        stream.sync()
 
 
 if __name__ == "__main__":

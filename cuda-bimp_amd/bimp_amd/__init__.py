@@ -24,7 +24,7 @@ OK, W_OVERFLOW, E_ARG, E_ALLOC, E_DEVICE, E_UNSUPPORTED = 0, 1, -1, -2, -3, -4
 MAP_DOUBLE, MAP_SINGLE, MAP_COMPLEX = 0, 1, 2
 FLAG_PROFILE = 1
 PRECISION_FP32, PRECISION_BF16, PRECISION_FP16 = 0, 1, 2
-DIRECT_MAX_EXTENT = 63
+DIRECT_MAX_KERNEL = 55
 
 # SURF_CUDA keypoint-matrix rows (utils.cpp:109-150)
 X_ROW, Y_ROW, LAPLACIAN_ROW, OCTAVE_ROW, SIZE_ROW, ANGLE_ROW, HESSIAN_ROW = range(7)
@@ -73,6 +73,9 @@ EXPORTED = [
     "bimp_device_count", "bimp_device_malloc", "bimp_device_free", "bimp_memcpy_h2d",
     "bimp_memcpy_d2h", "bimp_stream_create", "bimp_stream_destroy", "bimp_stream_sync",
     "bimp_time_detect", "bimp_copy_kernel", "bimp_synth_frame",
+    "bimp_batch_create", "bimp_batch_destroy", "bimp_batch_frame", "bimp_batch_run",
+    "bimp_batch_results", "bimp_batch_outputs", "bimp_batch_time", "bimp_batch_plan",
+    "bimp_batch_last_error",
 ]
 
 
@@ -115,6 +118,15 @@ def _load():
         "bimp_time_detect": (I, [P, P, SZ, P, I, I, P, C.POINTER(C.c_float)]),
         "bimp_copy_kernel": (I, [P, P, SZ, P]),
         "bimp_synth_frame": (I, [I, I, I, C.c_uint64, P, SZ]),
+        "bimp_batch_create": (I, [P, P, I, P, I, C.POINTER(Options), I, I, C.POINTER(P)]),
+        "bimp_batch_destroy": (I, [P]),
+        "bimp_batch_frame": (I, [P, I, C.POINTER(P), C.POINTER(SZ)]),
+        "bimp_batch_run": (I, [P, P]),
+        "bimp_batch_results": (I, [P, I, P, I, C.POINTER(I)]),
+        "bimp_batch_outputs": (I, [P, I, C.POINTER(P), C.POINTER(P)]),
+        "bimp_batch_time": (I, [P, I, P, C.POINTER(C.c_float), P]),
+        "bimp_batch_plan": (I, [P, C.c_char_p, I]),
+        "bimp_batch_last_error": (C.c_char_p, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -308,6 +320,73 @@ class CudaStuff:
         self._chk(lib.bimp_time_detect(self.ctx, d_img.ptr, pitch, d_out.ptr, cap, iters,
                                        stream.handle if stream else None, C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Batching form: frames of uneven sizes, one hipGraph launch per batch (bimp_batch_*)."""
+
+    def __init__(self, sizes, lambdas, options=None, cap=KP_MAX_POINTS, tune=False):
+        w = np.ascontiguousarray([s[0] for s in sizes], np.int32)
+        h = np.ascontiguousarray([s[1] for s in sizes], np.int32)
+        lam = np.ascontiguousarray(lambdas, np.float32)
+        opt = options if options is not None else default_options()
+        b = C.c_void_p()
+        st = lib.bimp_batch_create(_ptr(w), _ptr(h), len(sizes), _ptr(lam), lam.size,
+                                   C.byref(opt), int(cap), 1 if tune else 0, C.byref(b))
+        if st < 0:
+            raise BimpError(st, lib.bimp_status_string(st).decode())
+        self.b = b
+        self.sizes = [(int(a), int(c)) for a, c in sizes]
+        self.cap = int(cap)
+
+    def _chk(self, st):
+        if st < 0:
+            raise BimpError(st, lib.bimp_batch_last_error(self.b).decode() or
+                            lib.bimp_status_string(st).decode())
+        return st
+
+    def upload(self, i, img):
+        """Copy a host uint8 frame into frame slot i."""
+        w, h = self.sizes[i]
+        img = np.ascontiguousarray(img, np.uint8)
+        assert img.shape == (h, w)
+        d, pitch = C.c_void_p(), C.c_size_t()
+        self._chk(lib.bimp_batch_frame(self.b, i, C.byref(d), C.byref(pitch)))
+        if pitch.value == w:
+            self._chk(lib.bimp_memcpy_h2d(d, _ptr(img), img.nbytes))
+        else:
+            padded = np.zeros((h, pitch.value), np.uint8)
+            padded[:, :w] = img
+            self._chk(lib.bimp_memcpy_h2d(d, _ptr(padded), padded.nbytes))
+
+    def run(self, stream=None):
+        self._chk(lib.bimp_batch_run(self.b, stream.handle if stream else None))
+
+    def results(self, i):
+        out = np.zeros(max(self.cap, 1), KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        self._chk(lib.bimp_batch_results(self.b, i, _ptr(out), self.cap, C.byref(n)))
+        return out[:min(n.value, self.cap)].copy(), n.value
+
+    def time(self, iters, stream, per_frame=True):
+        ms = C.c_float(0)
+        fr = np.zeros(len(self.sizes), np.float32)
+        self._chk(lib.bimp_batch_time(self.b, int(iters), stream.handle, C.byref(ms),
+                                      _ptr(fr) if per_frame else None))
+        return ms.value, (fr if per_frame else None)
+
+    def plan(self):
+        buf = C.create_string_buffer(4096)
+        lib.bimp_batch_plan(self.b, buf, 4096)
+        return buf.value.decode()
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib.bimp_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        self.close()
 
 
 def get_keypoints(img, cs, cap=KP_MAX_POINTS):

@@ -33,22 +33,26 @@ void quadrature_bank(const ScaleGeom& g, float gamma, float sigma_ratio, int thr
 
 // Precision form: the 16 filters of one scale (even and odd kernel of each orientation,
 // o = 0..7, filter index 2*o and 2*o+1) flipped for convolution, scaled by 1/fscale and
-// rounded once to BF16 or FP16, laid out as the B operand of v_mfma_f32_16x16x32:
-//   frag[(ks*64 + lane)*8 + t] = B[32*ks + 8*(lane/16) + t][lane%16]
-// with K index k = u*extp + v (u, v < ext; zero elsewhere).  nsig <= 0 (or an extent that
-// reaches kW) keeps the whole kW x kW kernel Ke; nsig > 0 truncates to the odd extent
-// ceil(nsig*sigma) and the even filter becomes the carrier Ke + dc inside the window.  The
-// caller subtracts dc[f] times the kW x kW box sum of the image from filter f's response,
+// rounded once to BF16 or FP16.  nsig <= 0 (or an extent that reaches kW) keeps the whole
+// kW x kW kernel Ke; nsig > 0 truncates to the odd extent ceil(nsig*sigma) and the even filter
+// becomes the carrier Ke + dc inside the window.  The caller subtracts dc[f] times the
+// kW x kW box sum of the image from filter f's response,
 //   dc[f] = (fscale * sum(rounded taps of f) - sum(exact kW x kW kernel of f)) / kW^2,
 // so that a constant image gets the exact full kernel's response.
+//
+// Layout: the B operand of v_mfma_f32_16x16x32 for every tap block of 4 rows x 8 columns,
+//   frag[((i*nvb + vb)*64 + lane)*8 + t] = tap (u0 + lane/16, 8*vb + t) of filter lane%16,
+// with u0 = i - (rows - 1) for i < 4*nrho + rows - 1, nvb = ceil(ext/8),
+// nrho = ceil((rows - 1 + ext)/4), and zero outside 0 <= u, v < ext.  `rows` is the number of
+// output rows one wavefront accumulates (it sets the range of u0).
 struct DirectBank {
-    int ext = 0, R = 0, extp = 0, ksteps = 0;
+    int ext = 0, nvb = 0, nrho = 0;
     float fscale = 1.f;
     float dc[16] = {0};
     std::vector<uint16_t> frag;
 };
 void direct_bank(const ScaleGeom& g, float gamma, float sigma_ratio, float nsig, bool fp16,
-                 DirectBank& out);
+                 int rows, DirectBank& out);
 // Extent direct_bank would choose (without building the taps).
 int direct_extent(const ScaleGeom& g, float sigma_ratio, float nsig);
 uint16_t float_to_bf16(float f);  // round to nearest even

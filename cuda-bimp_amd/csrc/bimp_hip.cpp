@@ -17,91 +17,11 @@
 
 #include "../../include/bimp_hip.h"
 #include "bank.hpp"
+#include "context.hpp"
 #include "kernels.hpp"
 
-using bimpk::kNumOri;
-using bimpk::kUniqueOri;
+using namespace bimpi;
 
-namespace {
-
-enum Family {
-    F_INIT, F_COPY_IN, F_PYR, F_FWD_ROWS, F_FWD_COLS, F_INV_COLS, F_INV_ROWS, F_CELLS,
-    F_MAX, F_NMS_COUNT, F_NMS_SCAN, F_NMS_WRITE, F_BOX, F_DIRECT, F_COUNT
-};
-// Profile names follow the kernel symbols (size-templated kernels carry their length).
-const char* kFamilyNames[F_COUNT] = {"k_init",      "copy_in",    "k_pyr_down",  "k_fwd_rows",
-                                     "k_fwd_cols",  "k_inv_cols", "k_inv_rows",  "k_cells",
-                                     "k_max",       "k_nms_count", "k_nms_scan", "k_nms_write",
-                                     "k_box",       "k_direct"};
-
-struct Scale {
-    bimph::ScaleGeom g;
-    int Hc = 0, Wc = 0, ldc = 0;
-    bimpk::CellParams cp;
-    float* G = nullptr;    // kUniqueOri x FW x FH
-    float2* DT = nullptr;  // (FW/2+1) x FH
-    float* C = nullptr;    // 8 x Hc x ldc
-    float* D = nullptr;    // H x W
-    float* S = nullptr;    // H x W
-    // direct (low-precision) form
-    int precision = BIMP_PRECISION_FP32;
-    bimpk::DirectScale ds{};
-    void* frag = nullptr;  // B-operand fragments, ksteps x 64 lanes x 8 taps (16-bit)
-};
-
-// One step of a frame: an FFT resolution group or one direct-form scale.
-struct Step {
-    bool direct;
-    int index;  // group index or scale index
-};
-
-// Scales sharing pyramid level and transform size; each FFT stage runs once per group.
-struct Group {
-    int first, n;
-};
-
-struct ProfEv {
-    hipEvent_t a, b;
-    int fam, size;
-    double bytes;
-};
-
-struct ProfAcc {
-    int launches = 0;
-    double ms = 0, bytes = 0;
-};
-
-}  // namespace
-
-struct bimp_ctx {
-    int width = 0, height = 0;
-    bimp_options opt;
-    std::vector<Scale> sc;
-    std::vector<Group> groups;
-    std::vector<Step> steps;
-    int maxLevel = 0;
-    std::vector<uint8_t*> level;  // level[0] is the staging copy of a host frame
-    std::vector<int> levelW, levelH;
-    float2* R = nullptr;
-    float2* T = nullptr;
-    int* boxRows = nullptr;  // direct form with truncated taps: horizontal box sums
-    float* box = nullptr;    // and the kW x kW box sums
-    std::map<int, float2*> tw;
-    unsigned* maxslots = nullptr;
-    int* counter = nullptr;
-    int* blockCounts = nullptr;
-    int totalBlocks = 0;
-    bimpk::NmsScale* nmsTable = nullptr;
-    void* kpBuf = nullptr;
-    size_t kpBufBytes = 0;
-    bool maxValid = false;
-    std::string err;
-    // profiling
-    std::vector<ProfEv> ev;
-    size_t evUsed = 0;
-    std::map<std::string, ProfAcc> prof;
-    hipStream_t lastStream = nullptr;
-};
 
 namespace {
 
@@ -180,8 +100,6 @@ void free_all(bimp_ctx* c) {
         (void)hipFree(s.S);
         (void)hipFree(s.frag);
     }
-    (void)hipFree(c->boxRows);
-    (void)hipFree(c->box);
     for (auto* p : c->level) (void)hipFree(p);
     for (auto& kv : c->tw) (void)hipFree(kv.second);
     (void)hipFree(c->R);
@@ -206,12 +124,20 @@ int launch_cells_scale(bimp_ctx* c, size_t i, hipStream_t s) {
     return BIMP_OK;
 }
 
-// Pyramid, forward transforms (one launch per stage and resolution group), then per scale
-// the inverse column and row passes and, with `cells`, the end-stop/inhibition stencil.
-// Direct-form scales replace the transforms by one MFMA convolution launch (plus the box sums
-// when their taps are truncated).
-// Running a scale's consumers right after its producers keeps the intermediate T (one scale)
-// and the complex-cell maps resident in the 256 MB last-level cache.
+// Budget for the inverse-pass intermediate T: scales of one group share an inverse launch
+// while their T stays within it, so that T is consumed from the 256 MB last-level cache.
+constexpr size_t kTBudgetBytes = 160u << 20;
+
+int inverse_batch(size_t tstride_elems) {
+    const size_t per = tstride_elems * sizeof(float2);
+    return (int)std::max<size_t>(1, std::min<size_t>(bimpk::kMaxBatch, kTBudgetBytes / per));
+}
+
+// Pyramid, forward transforms (one launch per stage and resolution group), then per
+// sub-batch of scales the inverse column and row passes and, with `cells`, the
+// end-stop/inhibition stencil.  Direct-form scales replace the transforms by one MFMA
+// convolution launch.  Running the consumers right after their producers keeps the
+// intermediate T and the complex-cell maps resident in the 256 MB last-level cache.
 int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, bool cells,
                hipStream_t s) {
     const int W0 = c->width, H0 = c->height;
@@ -247,17 +173,11 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
             const Scale& sk = c->sc[i];
             const auto& g = sk.g;
             {
-                LaunchScope ls(c, s, F_BOX,
-                               (double)g.H * g.W + 4.0 * sk.Hc * sk.Wc);
-                HIPCHK(c, bimpk::launch_box(lv[g.level], lp[g.level], sk.ds, g.kX, c->boxRows,
-                                            c->box, s));
-            }
-            {
                 LaunchScope ls(c, s, F_DIRECT,
                                (double)g.H * g.W + 4.0 * kNumOri * sk.Hc * sk.Wc, sk.ds.ext);
                 HIPCHK(c, bimpk::launch_direct(lv[g.level], lp[g.level], sk.ds,
                                                sk.precision == BIMP_PRECISION_FP16, sk.frag,
-                                               c->box, sk.C, s));
+                                               sk.C, s));
             }
             if (cells) {
                 const int r = launch_cells_scale(c, (size_t)i, s);
@@ -290,31 +210,41 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
             LaunchScope ls(c, s, F_FWD_COLS, n * (8.0 * g.H * ncols + 8.0 * ncols * g.FH), g.FH);
             HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, rstride, g.H, ss, ncols, twH, s));
         }
-        for (int k = 0; k < gr.n; ++k) {
-            const Scale& sk = c->sc[gr.first + k];
-            bimpk::ScaleSet one;
-            one.n = 1;
-            one.kX[0] = sk.g.kX;
-            one.G[0] = sk.G;
-            one.DT[0] = sk.DT;
-            one.C[0] = sk.C;
-            const size_t tstride = (size_t)kNumOri * sk.Hc * g.FW;
+        // inverse passes over sub-batches of scales whose intermediate T fits the budget
+        const Scale& s0 = c->sc[gr.first];
+        const size_t tstride = (size_t)kNumOri * s0.Hc * g.FW;
+        const int nb = std::max(1, std::min(c->invBatch > 0 ? c->invBatch : inverse_batch(tstride),
+                                            (int)(c->tCap / tstride)));
+        for (int k0 = 0; k0 < gr.n; k0 += nb) {
+            bimpk::ScaleSet sub;
+            sub.n = std::min(nb, gr.n - k0);
+            for (int k = 0; k < sub.n; ++k) {
+                const Scale& sk = c->sc[gr.first + k0 + k];
+                sub.kX[k] = sk.g.kX;
+                sub.G[k] = sk.G;
+                sub.DT[k] = sk.DT;
+                sub.C[k] = sk.C;
+            }
+            const double m = sub.n;
             {
                 LaunchScope ls(c, s, F_INV_COLS,
-                               8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
-                                   8.0 * kNumOri * sk.Hc * g.FW,
+                               m * (8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
+                                    8.0 * kNumOri * s0.Hc * g.FW),
                                g.FH);
-                HIPCHK(c, bimpk::launch_inv_cols(g.FH, one, g.FW, sk.Hc, c->T, tstride, twH, s));
+                HIPCHK(c, bimpk::launch_inv_cols(g.FH, sub, g.FW, s0.Hc, c->T, tstride, twH, s));
             }
             {
                 LaunchScope ls(c, s, F_INV_ROWS,
-                               8.0 * kNumOri * sk.Hc * g.FW + 4.0 * kNumOri * sk.Hc * sk.Wc, g.FW);
-                HIPCHK(c, bimpk::launch_inv_rows(g.FW, one, c->T, tstride, sk.Hc, sk.Wc, sk.ldc,
+                               m * (8.0 * kNumOri * s0.Hc * g.FW + 4.0 * kNumOri * s0.Hc * s0.Wc),
+                               g.FW);
+                HIPCHK(c, bimpk::launch_inv_rows(g.FW, sub, c->T, tstride, s0.Hc, s0.Wc, s0.ldc,
                                                  twW, s));
             }
             if (cells) {
-                const int st = launch_cells_scale(c, (size_t)(gr.first + k), s);
-                if (st) return st;
+                for (int k = 0; k < sub.n; ++k) {
+                    const int st = launch_cells_scale(c, (size_t)(gr.first + k0 + k), s);
+                    if (st) return st;
+                }
             }
         }
     }
@@ -412,6 +342,13 @@ int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_dev
 
 }  // namespace
 
+int bimpi::run_frame(bimp_ctx* c, const uint8_t* d_img, size_t pitch, void* out, int cap,
+                     int matrix_ld, hipStream_t s) {
+    const int st = run_filter(c, d_img, pitch, 1, true, s);
+    if (st) return st;
+    return run_keypoints(c, out, cap, matrix_ld, s);
+}
+
 // ======================================================================= C ABI
 extern "C" {
 
@@ -474,8 +411,13 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             return BIMP_E_ARG;
         }
         if (c->opt.precision != BIMP_PRECISION_FP32 && l <= c->opt.direct_max_lambda &&
-            bimph::direct_extent(s.g, c->opt.sigma_ratio, c->opt.direct_sigmas) <= BIMP_DIRECT_MAX_EXTENT)
-            s.precision = c->opt.precision;
+            s.g.kW <= BIMP_DIRECT_MAX_KERNEL) {
+            bimpk::DirectScale d{};
+            d.kpad = s.g.kX;
+            d.kW = s.g.kW;
+            d.ext = bimph::direct_extent(s.g, c->opt.sigma_ratio, c->opt.direct_sigmas);
+            if (bimpk::direct_geometry(d) <= bimpk::kMaxLds) s.precision = c->opt.precision;
+        }
         if (s.precision == BIMP_PRECISION_FP32 &&
             (!bimpk::fft_size_supported(s.g.FH) || !bimpk::fft_size_supported(s.g.FW) ||
              s.g.FW % bimpk::col_batch(s.g.FH) != 0)) {
@@ -552,9 +494,9 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     for (const auto& gr : c->groups) {
         const Scale& s0 = c->sc[gr.first];
         rMax = std::max(rMax, (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
-        tMax = std::max(tMax, (size_t)kNumOri * s0.Hc * s0.g.FW);
+        const size_t ts = (size_t)kNumOri * s0.Hc * s0.g.FW;
+        tMax = std::max(tMax, ts * std::min(gr.n, inverse_batch(ts)));
     }
-    size_t boxRowsMax = 0, boxMax = 0;
     bool anyDirect = false;
     for (auto& s : c->sc) {
         const auto& g = s.g;
@@ -566,28 +508,23 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             anyDirect = true;
             bimph::DirectBank db;
             bimph::direct_bank(g, c->opt.gamma, c->opt.sigma_ratio, c->opt.direct_sigmas,
-                               s.precision == BIMP_PRECISION_FP16, db);
+                               s.precision == BIMP_PRECISION_FP16, bimpk::kDirRows, db);
             auto& d = s.ds;
             d.H = g.H;
             d.W = g.W;
             d.FH = g.FH;
             d.FW = g.FW;
             d.kpad = g.kX;
+            d.kW = g.kW;
             d.Hc = s.Hc;
             d.Wc = s.Wc;
             d.ldc = s.ldc;
-            d.R = db.R;
             d.ext = db.ext;
-            d.extp = db.extp;
-            d.ksteps = db.ksteps;
-            d.lwp = bimpk::direct_lwp(db.R, db.extp);
             d.fscale = db.fscale;
             d.iscale = c->opt.input_scale;
             for (int f = 0; f < 2 * kNumOri; ++f) d.dc[f] = db.dc[f];
-            if (bimpk::direct_lds_bytes(d) > bimpk::kDirectMaxLds)
+            if (bimpk::direct_geometry(d) > bimpk::kMaxLds || d.nvb != db.nvb || d.nrho != db.nrho)
                 return bail(BIMP_E_UNSUPPORTED, "direct-form tile exceeds LDS");
-            boxRowsMax = std::max(boxRowsMax, (size_t)(s.Hc + 2 * g.kX) * s.Wc);
-            boxMax = std::max(boxMax, (size_t)s.Hc * s.Wc);
             if (dalloc((uint16_t**)&s.frag, db.frag.size()) != hipSuccess)
                 return bail(BIMP_E_ALLOC, "direct taps");
             if (hipMemcpy(s.frag, db.frag.data(), db.frag.size() * sizeof(uint16_t),
@@ -610,10 +547,9 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         if (hipMemset(s.C, 0, (size_t)kNumOri * s.Hc * s.ldc * sizeof(float)) != hipSuccess)
             return bail(BIMP_E_DEVICE, "memset");
     }
+    c->tCap = tMax;
     if ((rMax && dalloc(&c->R, rMax) != hipSuccess) || (tMax && dalloc(&c->T, tMax) != hipSuccess))
         return bail(BIMP_E_ALLOC, "scratch");
-    if (boxMax && (dalloc(&c->boxRows, boxRowsMax) != hipSuccess || dalloc(&c->box, boxMax) != hipSuccess))
-        return bail(BIMP_E_ALLOC, "box sums");
     if (anyDirect) {
         if (hipError_t e = bimpk::prepare_direct(); e != hipSuccess)
             return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));

@@ -7,17 +7,23 @@
 // or FP16 with FP32 accumulation (v_mfma_f32_16x16x32_{bf16,f16}): taps scaled by 1/fscale
 // and rounded once, image samples entering as their uint8 values (exact in both formats).
 // With truncated taps the even filter is the carrier Ke_o + dc_o inside the window.  Either
-// way each filter's response is corrected by -dc[f] * B, B the kW x kW box sum of Pd computed
-// exactly in integers (k_box_rows / k_box_cols), with dc[f] chosen so that a constant image
-// gets the exact full kernel's response: rounding and truncation then leave no DC leak.  The
-// 255/65536 input scale and fscale are applied to the FP32 accumulators.
+// way each filter's response is corrected by -dc[f] * B, B the kW x kW box sum of Pd, with
+// dc[f] chosen so that a constant image gets the exact full kernel's response: rounding and
+// truncation then leave no DC leak.  B is summed in FP32 from the staged tile; every partial
+// sum is an integer below 2^24, so it is exact.  The 255/65536 input scale and fscale are
+// applied to the FP32 accumulators.
 //
-// GEMM shape per workgroup (8 wavefronts, output tile 16 rows x 64 columns): M = pixels in
-// groups of 16 consecutive columns, N = 16 filters (even/odd x 8 orientations), K = taps
-// (u, v) with v padded to a multiple of 8 so that the 8 taps a lane holds lie in one image
-// row.  The image tile sits in LDS as four copies shifted by 0..3 samples, which makes every
-// lane's 8-sample window two aligned 8-byte reads.
+// GEMM shape: M = 16 pixels (consecutive columns), N = 16 filters (even/odd x 8
+// orientations), K = 32 taps = 4 tap rows x 8 tap columns, one tap row per 16-lane group.
+// A K-step is keyed by an aligned block of 4 image rows: the image fragment of rows
+// 4*rho .. 4*rho+3 serves every output row j of the wavefront, with the tap block
+// u0 = 4*rho - j (filter fragments precomputed for every u0).  So each fragment read from
+// LDS feeds kDirRows x kDirGroups MFMAs and each filter fragment kDirGroups of them.
+// The image tile sits in LDS as four copies shifted by 0..3 samples: every lane's 8-sample
+// window is then two aligned 8-byte reads.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 
@@ -50,69 +56,43 @@ struct Mfma<_Float16> {
     }
 };
 
-// Row of the periodic padded image (padDataClampToBorder rule, convolutionFFT2D.cuh:56-85)
+// Row (or column) of the periodic padded image (padDataClampToBorder rule,
+// convolutionFFT2D.cuh:56-85): index i of the F-periodic domain, data size n, border k.
 __device__ __forceinline__ int pad_index(int i, int n, int F, int k) {
     i %= F;
     if (i < 0) i += F;
     return i < n ? i : (i < n + k ? n - 1 : 0);
 }
 
-constexpr int kDirTY = 16, kDirTX = 64, kDirWaves = 8;
-
-// Horizontal box sums over kW = 2k+1 columns of the periodic padded uint8 image, for the
-// rows y in [-k, Hc + k) (stored at y + k) and columns x < Wc.
-__global__ void __launch_bounds__(256) k_box_rows(const uint8_t* __restrict__ img, size_t pitch,
-                                                  int H, int W, int FH, int FW, int k, int Hc,
-                                                  int Wc, int* __restrict__ hs) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;  // 0 .. Hc + 2k - 1
-    if (x >= Wc) return;
-    const uint8_t* row = img + (size_t)pad_index(r - k, H, FH, k) * pitch;
-    int acc = 0;
-    for (int v = -k; v <= k; ++v) acc += row[pad_index(x - v, W, FW, k)];
-    hs[(size_t)r * Wc + x] = acc;
-}
-
-__global__ void __launch_bounds__(256) k_box_cols(const int* __restrict__ hs, int k, int Hc,
-                                                  int Wc, float* __restrict__ box) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= Wc) return;
-    int acc = 0;
-    for (int u = -k; u <= k; ++u) acc += hs[(size_t)(y - u + k) * Wc + x];
-    box[(size_t)y * Wc + x] = (float)acc;
-}
-
 template <class T>
 __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
         const uint8_t* __restrict__ img, size_t pitch, DirectScale ds,
-        const typename Mfma<T>::v8* __restrict__ bfrag, const float* __restrict__ box,
-        float* __restrict__ Cm) {
+        const typename Mfma<T>::v8* __restrict__ bfrag, float* __restrict__ Cm) {
     typedef typename Mfma<T>::v8 v8;
     typedef typename Mfma<T>::v4 v4;
+    constexpr int M = kDirRows, NCG = kDirGroups, TY = kDirTY, TX = kDirTX;
     extern __shared__ unsigned char smem[];
     T* copies = reinterpret_cast<T*>(smem);
-    const int R = ds.R, extp = ds.extp;
-    const int LH = kDirTY + 2 * R;
-    const int LWP = ds.lwp;  // multiple of 4, >= TX + extp + 8
-    const int csize = LH * LWP;
-    const int x0 = blockIdx.x * kDirTX, y0 = blockIdx.y * kDirTY;
-    const int tid = threadIdx.x;
+    const int LH = ds.LH, LWP = ds.lwp, csize = LH * LWP;
+    float* vs = reinterpret_cast<float*>(smem + (size_t)4 * csize * sizeof(T));
+    const int k = ds.kpad, kW = ds.kW;
+    const int TWi = TX + 2 * k;  // columns of the image window the tile needs
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
-    // stage the tile (rows y0-R.., columns x0-R..) as four shifted copies: tile sample c goes
-    // to position c - s of copy s.  Taps past the truncated extent (v >= ext) meet zero
-    // filter entries, so every position that can be read is finite (zero past the tile).
-    const int wave = tid >> 6, lane = tid & 63;
-    const int TW = kDirTX + 2 * R;
-    const bool inside = y0 - R >= 0 && y0 + kDirTY + R <= ds.H && x0 - R >= 0 && x0 + TW - R <= ds.W;
+    // Stage rows y0-k .. and columns x0-k .. of Pd as four shifted copies (tile sample c at
+    // position c - s of copy s).  Positions past the image window hold zero; they only meet
+    // zero taps.
+    const bool inside = y0 - k >= 0 && y0 + TY + k <= ds.H && x0 - k >= 0 && x0 + TX + k <= ds.W;
     for (int r = wave; r < LH; r += kDirWaves) {
-        const int gy = inside ? y0 - R + r : pad_index(y0 - R + r, ds.H, ds.FH, ds.kpad);
+        const bool rowin = r < TY + 2 * k;
+        const int gy = !rowin ? 0 : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
         const uint8_t* row = img + (size_t)gy * pitch;
         T* dst = copies + r * LWP;
         for (int c = lane; c < LWP; c += 64) {
             T v = T(0.f);
-            if (c < TW) {
-                const int gx = inside ? x0 - R + c : pad_index(x0 - R + c, ds.W, ds.FW, ds.kpad);
+            if (rowin && c < TWi) {
+                const int gx = inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k);
                 v = T((float)row[gx]);
             }
 #pragma unroll
@@ -122,69 +102,123 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     }
     __syncthreads();
 
-    const int p = lane & 15, kb = lane >> 4;
-    f32x4 acc[8];
-#pragma unroll
-    for (int g = 0; g < 8; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // this lane's taps in K-step 0: k0 = 8*kb -> (u, v0)
-    int u = 0, v0 = 8 * kb;
-    while (v0 >= extp) {
-        v0 -= extp;
-        ++u;
+    // kW x kW box sums of the tile's output pixels: vertical running sums over kW rows into
+    // vs[r][c] (two segments of TY/2 rows per column), then horizontal ones, stored back into
+    // vs as a TY x TX array.
+    if (tid < 2 * TWi) {
+        const int seg = tid >= TWi ? 1 : 0;
+        const int c = tid - seg * TWi, r0 = seg * (TY / 2);
+        const T* col = copies + c;
+        float sum = 0.f;
+        for (int u = 0; u < kW; ++u) sum += (float)col[(r0 + u) * LWP];
+        vs[r0 * ds.vsp + c] = sum;
+        for (int r = 1; r < TY / 2; ++r) {
+            sum += (float)col[(r0 + r + kW - 1) * LWP] - (float)col[(r0 + r - 1) * LWP];
+            vs[(r0 + r) * ds.vsp + c] = sum;
+        }
     }
-#pragma unroll 1
-    for (int ks = 0; ks < ds.ksteps; ++ks) {
-        const v8 b = bfrag[(size_t)ks * 64 + lane];
-        const int uu = u < ds.ext ? u : ds.ext - 1;  // K padding past the last row: zero taps
+    __syncthreads();
+    float bsum[8];
+    {
+        const int r = tid >> 4, xs = (tid & 15) * 8;
+        const float* vr = vs + r * ds.vsp + xs;
+        float sum = 0.f;
+        for (int v = 0; v < kW; ++v) sum += vr[v];
+        bsum[0] = sum;
 #pragma unroll
-        for (int g = 0; g < 8; ++g) {
-            const int lr = 2 * wave + (g >> 2) + uu;          // tile row
-            const int c = 16 * (g & 3) + p + v0;               // tile column of the window
-            const int s = c & 3;
-            const T* src = copies + s * csize + lr * LWP + (c - s);
-            const v4 lo = *reinterpret_cast<const v4*>(src);
-            const v4 hi = *reinterpret_cast<const v4*>(src + 4);
-            v8 a;
-            a[0] = lo[0]; a[1] = lo[1]; a[2] = lo[2]; a[3] = lo[3];
-            a[4] = hi[0]; a[5] = hi[1]; a[6] = hi[2]; a[7] = hi[3];
-            acc[g] = Mfma<T>::run(a, b, acc[g]);
+        for (int i = 1; i < 8; ++i) {
+            sum += vr[i + kW - 1] - vr[i - 1];
+            bsum[i] = sum;
         }
-        v0 += 32;
-        while (v0 >= extp) {
-            v0 -= extp;
-            ++u;
-        }
+    }
+    __syncthreads();
+    {
+        const int r = tid >> 4, xs = (tid & 15) * 8;
+        float4* dstb = reinterpret_cast<float4*>(vs + r * TX + xs);
+        dstb[0] = make_float4(bsum[0], bsum[1], bsum[2], bsum[3]);
+        dstb[1] = make_float4(bsum[4], bsum[5], bsum[6], bsum[7]);
     }
 
-    // epilogue: lane holds filter j = lane%16 for pixels 4*(lane/16) .. +3 of each group;
-    // the odd filter of orientation j/2 sits in the neighbouring lane
-    const int j = lane & 15;
-    const int o = j >> 1;
+    // Main loop.  Lane (p, kb): pixel column p of each 16-column group, tap row kb of the
+    // K-step.  The copy index s = (off + p) & 3 is fixed per lane.
+    const int p = lane & 15, kb = lane >> 4;
+    const int s = (ds.off + p) & 3;
+    const T* lanebase = copies + s * csize + (ds.off + p - s) + (wave * M + ds.off + kb) * LWP;
+    f32x4 acc[M][NCG];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Software-pipelined over the K-steps it = rho * nvb + vb: the image and filter
+    // fragments of step it + 1 are in flight while the MFMAs of step it run.
+    const int nvb = ds.nvb, nsteps = ds.nrho * nvb;
+    auto load_a = [&](int it, v8* a) {
+        const int rho = it / nvb, vb = it - rho * nvb;
+        const T* rowp = lanebase + 4 * rho * LWP + 8 * vb;
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) {
+            const v4 lo = *reinterpret_cast<const v4*>(rowp + 16 * g);
+            const v4 hi = *reinterpret_cast<const v4*>(rowp + 16 * g + 4);
+            a[g] = v8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+    };
+    auto load_b = [&](int it, v8* b) {
+        const int rho = it / nvb, vb = it - rho * nvb;
+        // filter fragments of u0 = 4*rho - j sit at row index u0 + M - 1
+        const v8* bro = bfrag + ((size_t)(4 * rho + M - 1) * nvb + vb) * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < M; ++j) b[j] = bro[-j * nvb * 64];
+    };
+    v8 a[NCG], b[M];
+    load_a(0, a);
+    load_b(0, b);
+#pragma unroll 1
+    for (int it = 0; it < nsteps; ++it) {
+        v8 an[NCG], bn[M];
+        const int nx = it + 1 < nsteps ? it + 1 : it;
+        load_a(nx, an);
+        load_b(nx, bn);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int g = 0; g < NCG; ++g) acc[j][g] = Mfma<T>::run(a[g], b[j], acc[j][g]);
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) a[g] = an[g];
+#pragma unroll
+        for (int j = 0; j < M; ++j) b[j] = bn[j];
+    }
+    __syncthreads();  // the box sums stored before the main loop are read below
+
+    // Epilogue: lane holds filter jf = lane%16 for pixels 4*kb .. 4*kb+3 of each group; the
+    // odd filter of orientation jf/2 sits in the neighbouring lane.
+    const int jf = lane & 15, o = jf >> 1;
     const float dce = ds.dc[2 * o], dco = ds.dc[2 * o + 1];
 #pragma unroll
-    for (int g = 0; g < 8; ++g) {
-        const int y = y0 + 2 * wave + (g >> 2);
-        const int xb = x0 + 16 * (g & 3) + 4 * (lane >> 4);
-        float c4[4];
+    for (int j = 0; j < M; ++j) {
+        const int yl = wave * M + j, y = y0 + yl;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const float mine = acc[g][t];
-            const float other = __shfl_xor(mine, 1, 64);
-            const int x = xb + t;
-            float bx = 0.f;
-            if (y < ds.Hc && x < ds.Wc) bx = box[(size_t)y * ds.Wc + x];
-            const float e = ds.fscale * ((j & 1) ? other : mine) - dce * bx;
-            const float od = ds.fscale * ((j & 1) ? mine : other) - dco * bx;
-            c4[t] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
-        }
-        if (!(j & 1) && y < ds.Hc) {
-            float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + xb;
-            if (xb + 3 < ds.Wc) {
-                *reinterpret_cast<float4*>(out) = make_float4(c4[0], c4[1], c4[2], c4[3]);
-            } else {
+        for (int g = 0; g < NCG; ++g) {
+            const int xl = 16 * g + 4 * kb, x = x0 + xl;
+            const float4 bx = *reinterpret_cast<const float4*>(vs + yl * TX + xl);
+            const float bxs[4] = {bx.x, bx.y, bx.z, bx.w};
+            float c4[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (xb + t < ds.Wc) out[t] = c4[t];
+            for (int t = 0; t < 4; ++t) {
+                const float mine = acc[j][g][t];
+                const float other = __shfl_xor(mine, 1, 64);
+                const float e = ds.fscale * ((jf & 1) ? other : mine) - dce * bxs[t];
+                const float od = ds.fscale * ((jf & 1) ? mine : other) - dco * bxs[t];
+                c4[t] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
+            }
+            if (!(jf & 1) && y < ds.Hc) {
+                float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x;
+                if (x + 3 < ds.Wc) {
+                    *reinterpret_cast<float4*>(out) = make_float4(c4[0], c4[1], c4[2], c4[3]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (x + t < ds.Wc) out[t] = c4[t];
+                }
             }
         }
     }
@@ -192,41 +226,43 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
 
 }  // namespace
 
-size_t direct_lds_bytes(const DirectScale& ds) {
-    return (size_t)4 * (kDirTY + 2 * ds.R) * ds.lwp * 2;
+size_t direct_geometry(DirectScale& d) {
+    const int R = d.ext / 2;
+    const int extp = (d.ext + 7) / 8 * 8;
+    d.off = d.kpad - R;
+    d.nvb = extp / 8;
+    d.nrho = (kDirRows - 1 + d.ext + 3) / 4;
+    const int k = d.kpad;
+    d.LH = std::max(kDirTY + 2 * k, (kDirTY - kDirRows) + d.off + 4 * d.nrho);
+    const int twi = kDirTX + 2 * k;
+    const int tws = std::max(twi, d.off + kDirTX + extp - 1);
+    d.lwp = (tws + 4 + 3) / 4 * 4;
+    d.vsp = twi;
+    return (size_t)4 * d.LH * d.lwp * 2 + (size_t)kDirTY * d.vsp * sizeof(float);
 }
-
-int direct_lwp(int R, int extp) { return (kDirTX + 2 * R + extp + 8 + 3) / 4 * 4; }
 
 hipError_t prepare_direct() {
     hipError_t e = hipFuncSetAttribute((const void*)k_direct<__bf16>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDirectMaxLds);
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_direct<_Float16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDirectMaxLds);
-}
-
-hipError_t launch_box(const uint8_t* img, size_t pitch, const DirectScale& ds, int kfull,
-                      int* hs, float* box, hipStream_t s) {
-    dim3 b(256);
-    dim3 g1((ds.Wc + 255) / 256, ds.Hc + 2 * kfull);
-    k_box_rows<<<g1, b, 0, s>>>(img, pitch, ds.H, ds.W, ds.FH, ds.FW, kfull, ds.Hc, ds.Wc, hs);
-    dim3 g2((ds.Wc + 255) / 256, ds.Hc);
-    k_box_cols<<<g2, b, 0, s>>>(hs, kfull, ds.Hc, ds.Wc, box);
-    return hipGetLastError();
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
 }
 
 hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectScale& ds, bool fp16,
-                         const void* bfrag, const float* box, float* C, hipStream_t s) {
-    const size_t lds = direct_lds_bytes(ds);
-    if (lds > kDirectMaxLds) return hipErrorInvalidValue;
+                         const void* bfrag, float* C, hipStream_t s) {
+    DirectScale d = ds;
+    const size_t lds = direct_geometry(d);
+    // the geometry must be the one the filter fragments were built for
+    if (lds > kMaxLds || d.off != ds.off || d.nvb != ds.nvb || d.nrho != ds.nrho ||
+        d.LH != ds.LH || d.lwp != ds.lwp || d.vsp != ds.vsp ||
+        2 * (kDirTX + 2 * ds.kpad) > kDirWaves * 64 || ds.kW != 2 * ds.kpad + 1 || ds.off < 0)
+        return hipErrorInvalidValue;
     dim3 grid((ds.Wc + kDirTX - 1) / kDirTX, (ds.Hc + kDirTY - 1) / kDirTY);
     if (fp16)
-        k_direct<_Float16><<<grid, kDirWaves * 64, lds, s>>>(img, pitch, ds,
-                                                             (const f16x8*)bfrag, box, C);
+        k_direct<_Float16><<<grid, kDirWaves * 64, lds, s>>>(img, pitch, ds, (const f16x8*)bfrag, C);
     else
-        k_direct<__bf16><<<grid, kDirWaves * 64, lds, s>>>(img, pitch, ds, (const bf16x8*)bfrag,
-                                                           box, C);
+        k_direct<__bf16><<<grid, kDirWaves * 64, lds, s>>>(img, pitch, ds, (const bf16x8*)bfrag, C);
     return hipGetLastError();
 }
 

@@ -307,7 +307,6 @@ struct MapView {
 // unchanged (so does a -0 from the clamps).
 constexpr int kCellTW = 64, kCellTH = 16, kCellTY = 8, kCellRows = kCellTH / kCellTY;
 // dynamic LDS limit per workgroup (160 KiB per CU, minus room for static LDS)
-constexpr size_t kMaxLds = 160 * 1024 - 1024;
 
 struct CellTileOffsets {
     // LDS element offsets relative to the pixel's own position in a staged tile
@@ -741,6 +740,10 @@ __global__ void k_init(unsigned* maxslots, int nslots, int* counter) {
     if (i == 0) *counter = 0;
 }
 
+__global__ void k_store_count(const int* __restrict__ counter, int* __restrict__ dst) {
+    if (threadIdx.x == 0) dst[0] = counter[0];
+}
+
 __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -1004,6 +1007,11 @@ hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_block
                             void* out, int cap, int matrix_ld, hipStream_t s) {
     k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, maxslots, thresh,
                                                      block_offsets, out, cap, matrix_ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_count(const int* counter, int* dst, hipStream_t s) {
+    k_store_count<<<1, 64, 0, s>>>(counter, dst);
     return hipGetLastError();
 }
 

@@ -38,24 +38,28 @@ struct NmsScale {
     int slot;           // index of the scale's max slot
 };
 
+// Dynamic LDS a workgroup may request (160 KB per CU less the runtime's reserve).
+constexpr size_t kMaxLds = 160 * 1024 - 1024;
+
 // One scale of the precision form (direct MFMA convolution, see direct.hip).
+// Workgroup tile: kDirWaves wavefronts x kDirRows output rows, kDirGroups x 16 columns.
+constexpr int kDirRows = 4, kDirGroups = 8, kDirWaves = 8;
+constexpr int kDirTY = kDirWaves * kDirRows, kDirTX = kDirGroups * 16;
 struct DirectScale {
-    int H, W, FH, FW, kpad;  // data size, transform size, clamp-padding width (full kernel kX)
+    int H, W, FH, FW;        // data size, transform size (the periodic domain)
+    int kpad, kW;            // kernel centre kX (clamp padding, box half-width) and width
     int Hc, Wc, ldc;         // complex-cell map region and pitch
-    int R, ext, extp;        // truncated half-extent, extent (odd), row length padded to 8
-    int ksteps, lwp;         // K steps of 32 taps; LDS row pitch of the staged tile
+    int off, ext, nvb, nrho; // tap window offset kX - R, extent, 8-column blocks, 4-row blocks
+    int LH, lwp, vsp;        // staged rows, LDS row pitch (elements), box-sum row pitch (floats)
     float fscale, iscale;    // filter scale, input scale (255/65536)
     float dc[2 * kNumOri];   // box coefficients, filter 2o even, 2o+1 odd
 };
 
-constexpr size_t kDirectMaxLds = 150 * 1024;
-size_t direct_lds_bytes(const DirectScale& ds);
-int direct_lwp(int R, int extp);
+// Fills off/nvb/nrho/LH/lwp/vsp from kpad, kW, ext; returns the dynamic LDS bytes.
+size_t direct_geometry(DirectScale& ds);
 hipError_t prepare_direct();
-hipError_t launch_box(const uint8_t* img, size_t pitch, const DirectScale& ds, int kfull,
-                      int* hs, float* box, hipStream_t s);
 hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectScale& ds, bool fp16,
-                         const void* bfrag, const float* box, float* C, hipStream_t s);
+                         const void* bfrag, float* C, hipStream_t s);
 
 constexpr int kNmsThreads = 256;
 constexpr int kNmsPerThread = 16;
@@ -92,6 +96,8 @@ hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_block
                             const unsigned* maxslots, double thresh, const int* block_offsets,
                             void* out, int cap, int matrix_ld, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+// dst[0] = counter[0] on the stream (a kernel, so that it can sit in a captured graph).
+hipError_t launch_store_count(const int* counter, int* dst, hipStream_t s);
 
 // column-kernel batch (columns per workgroup) for transform length n
 int col_batch(int n);

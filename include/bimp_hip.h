@@ -82,12 +82,12 @@ typedef struct bimp_options {
 /* Arithmetic of the Gabor bank.  FP32: FFT convolution in FP32 (the reference's form).
  * BF16 / FP16: direct spatial convolution on the matrix cores, uint8 samples times taps
  * rounded once to BF16 / FP16, accumulated in FP32; used for every scale with
- * lambda <= direct_max_lambda whose kernel extent is at most BIMP_DIRECT_MAX_EXTENT, the
+ * lambda <= direct_max_lambda whose kernel width kW is at most BIMP_DIRECT_MAX_KERNEL, the
  * others keep the FP32 FFT form. */
 #define BIMP_PRECISION_FP32 0
 #define BIMP_PRECISION_BF16 1
 #define BIMP_PRECISION_FP16 2
-#define BIMP_DIRECT_MAX_EXTENT 63
+#define BIMP_DIRECT_MAX_KERNEL 55
 
 typedef struct bimp_scale_info {
     float lambda2;   /* requested wavelength (full resolution) */
@@ -101,7 +101,7 @@ typedef struct bimp_scale_info {
     int fft_h, fft_w;
     int map_h, map_w, map_pitch; /* stored complex-cell region and its row pitch (floats) */
     int precision;               /* BIMP_PRECISION_* this scale runs in */
-    int tap_extent;              /* direct forms: convolved kernel extent (odd), else kW */
+    int tap_extent;              /* direct forms: convolved tap extent (odd), else kW */
 } bimp_scale_info;
 
 typedef struct bimp_ctx bimp_ctx;
@@ -211,6 +211,36 @@ int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t str
  * N(0,5^2) noise, uint8.  Host memory. */
 int bimp_synth_frame(int width, int height, int nshapes, uint64_t seed, uint8_t* out,
                      size_t pitch);
+
+/* ---- Batching form --------------------------------------------------------------------
+ * Many frames of uneven sizes in one launch.  The reference has no batch entry: its caller
+ * runs getKeypoints frame by frame (example_bimp.cpp:48-54, cuda_bimp.cu:109-140); a batch
+ * gives every frame the same result as bimp_detect_u8 on a context of that frame's size.
+ * The batch owns one device input slot per frame (fill it, then run), per-frame keypoint
+ * outputs (cap records each, (scale, row, column) order) and per-frame counts.  The whole
+ * batch is one hipGraph: one branch per context, frames of one size spread over that size's
+ * contexts.  With tune != 0, creation times the candidate plans on the device (contexts per
+ * frame size, inverse-pass sub-batch per context) and keeps the fastest. */
+typedef struct bimp_batch bimp_batch;
+
+int bimp_batch_create(const int* widths, const int* heights, int nframes, const float* lambdas,
+                      int n, const bimp_options* opt, int cap, int tune, bimp_batch** out);
+int bimp_batch_destroy(bimp_batch* batch);
+/* Device input slot of frame i (uint8, row pitch *pitch bytes). */
+int bimp_batch_frame(bimp_batch* batch, int i, uint8_t** d_frame, size_t* pitch);
+/* Launch the batch graph on `stream` (no host synchronisation). */
+int bimp_batch_run(bimp_batch* batch, bimp_stream_t stream);
+/* After the stream has finished: frame i's keypoints (min(count, cap) records) and count. */
+int bimp_batch_results(bimp_batch* batch, int i, bimp_keypoint* out, int cap, int* count);
+/* Device outputs of frame i (cap records, then the count as one int). */
+int bimp_batch_outputs(bimp_batch* batch, int i, bimp_keypoint** d_keypoints, int** d_count);
+/* Time `iters` batch launches (elapsed milliseconds per batch) and, when frame_ms is not NULL,
+ * each frame alone as its own graph launch (milliseconds, nframes entries). */
+int bimp_batch_time(bimp_batch* batch, int iters, bimp_stream_t stream, float* batch_ms,
+                    float* frame_ms);
+/* Human-readable plan (contexts and sub-batches per frame size), NUL-terminated. */
+int bimp_batch_plan(const bimp_batch* batch, char* buf, int cap);
+const char* bimp_batch_last_error(const bimp_batch* batch);
 
 #ifdef __cplusplus
 }

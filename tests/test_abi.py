@@ -75,6 +75,21 @@ def test_errors_are_status_codes():
         with pytest.raises(B.BimpError) as e:
             B.CudaStuff(64, 48, [4.0])
         assert e.value.status == B.E_DEVICE
+        with pytest.raises(B.BimpError) as e:
+            B.Batch([(64, 48), (32, 32)], [4.0])
+        assert e.value.status == B.E_DEVICE
+    # precision options are validated before any device work
+    opt.precision = 7
+    assert B.lib.bimp_create(64, 64, B._ptr(lam), 1, C.byref(opt), C.byref(ctx)) == B.E_ARG
+    opt.precision = B.PRECISION_BF16
+    opt.direct_sigmas = -1.0
+    assert B.lib.bimp_create(64, 64, B._ptr(lam), 1, C.byref(opt), C.byref(ctx)) == B.E_ARG
+    # batch arguments
+    b = C.c_void_p()
+    w = np.array([64], np.int32)
+    assert B.lib.bimp_batch_create(B._ptr(w), B._ptr(w), 0, B._ptr(lam), 1, None, 10, 0,
+                                   C.byref(b)) == B.E_ARG
+    assert B.lib.bimp_batch_destroy(None) == B.E_ARG
 
 
 def test_facade_header_compiles(tmp_path):

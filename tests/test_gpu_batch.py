@@ -1,0 +1,101 @@
+"""GPU tests of the batching form (bimp_batch_*): one hipGraph launch over frames of uneven
+sizes gives every frame exactly the keypoints bimp_detect_u8 gives it alone, whatever plan
+the tuner picks, and replays are identical."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from kpmatch import bits
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(160, 120), (96, 80), (160, 120), (200, 150), (96, 80), (320, 240), (160, 120)]
+
+
+@pytest.fixture(scope="module")
+def B():
+    import bimp_amd
+    if bimp_amd.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP library")
+    return bimp_amd
+
+
+def _frames():
+    return [O.synth_frame(w, h, 10, 500 + i) for i, (w, h) in enumerate(SIZES)]
+
+
+def _single(B, frames, options=None):
+    ref = {}
+    for i, ((w, h), img) in enumerate(zip(SIZES, frames)):
+        cs = B.CudaStuff(w, h, O.BASE_BANK, options)
+        ref[i] = cs.detect(img)
+    return ref
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_equals_single_frame_detect(B, tune):
+    frames = _frames()
+    ref = _single(B, frames)
+    bt = B.Batch(SIZES, O.BASE_BANK, tune=tune)
+    print("plan:", bt.plan(), flush=True)
+    for i, img in enumerate(frames):
+        bt.upload(i, img)
+    s = B.Stream()
+    bt.run(s)
+    s.sync()
+    for i in range(len(SIZES)):
+        got, n = bt.results(i)
+        want, wn = ref[i]
+        assert n == wn, (i, n, wn)
+        assert np.array_equal(bits(got), bits(want)), i
+    bt.close()
+
+
+def test_batch_replays_identical_and_timed(B):
+    frames = _frames()
+    bt = B.Batch(SIZES, O.BASE_BANK)
+    for i, img in enumerate(frames):
+        bt.upload(i, img)
+    s = B.Stream()
+    bt.run(s)
+    s.sync()
+    first = [bt.results(i) for i in range(len(SIZES))]
+    ms, per = bt.time(3, s)
+    assert ms > 0 and np.all(per > 0)
+    for i in range(len(SIZES)):
+        got, n = bt.results(i)
+        assert n == first[i][1] and np.array_equal(bits(got), bits(first[i][0]))
+    bt.close()
+
+
+def test_batch_precision_form(B):
+    """The batch carries the context options: a BF16 batch equals BF16 single-frame runs."""
+    opt = B.default_options()
+    opt.precision = B.PRECISION_BF16
+    frames = _frames()
+    ref = _single(B, frames, opt)
+    bt = B.Batch(SIZES, O.BASE_BANK, options=opt)
+    for i, img in enumerate(frames):
+        bt.upload(i, img)
+    s = B.Stream()
+    bt.run(s)
+    s.sync()
+    for i in range(len(SIZES)):
+        got, n = bt.results(i)
+        assert n == ref[i][1] and np.array_equal(bits(got), bits(ref[i][0])), i
+    bt.close()
+
+
+def test_batch_overflow_reports_total(B):
+    frames = _frames()
+    bt = B.Batch(SIZES[:2], O.BASE_BANK, cap=5)
+    for i in range(2):
+        bt.upload(i, frames[i])
+    s = B.Stream()
+    bt.run(s)
+    s.sync()
+    ref = _single(B, frames)
+    got, n = bt.results(0)
+    assert n == ref[0][1] and len(got) == min(5, n)
+    assert np.array_equal(bits(got), bits(ref[0][0][:len(got)]))
+    bt.close()
