@@ -25,6 +25,11 @@ bool derive_scale(float lambda2, int width, int height, float pyr_threshold, flo
 // dc_out, when given, receives the subtracted DC offset evensum / fs^2.
 void gabor_taps(const ScaleGeom& g, int o, float gamma, float sigma_ratio, float* ke, float* ko,
                 float* dc_out = nullptr);
+// Derivatives of the taps of orientation o with respect to sigma_ratio (sigma = ratio *
+// lambda) and gamma, evaluated in FP64 on the same grid: d(Ke)/d., d(Ko)/d. (the DC term of Ke
+// differentiated too).  Arrays of kW*kW, row-major.
+void gabor_taps_grad(const ScaleGeom& g, int o, float gamma, float sigma_ratio, double* dke_dr,
+                     double* dko_dr, double* dke_dg, double* dko_dg);
 // Quadrature spectra G_o(ky, kx) = Re[DFT2(Ke + i*Ko)] / (FH*FW) for o = 0..4, stored
 // column-major: out[(o*FW + kx)*FH + ky].  DFT(Ke) is real and DFT(Ko) imaginary (Ke even,
 // Ko odd), so the real part carries the whole quadrature pair.

@@ -1,0 +1,367 @@
+// diff.cpp -- host side of the differentiation form (C ABI bimp_diff_*, include/bimp_hip.h).
+//
+// A diff context holds, per scale, the bank spectra Qhat_o of the current Gabor parameters
+// (sigma ratio, gamma) and the tap derivatives dK/dtheta; per frame it recomputes the complex
+// responses instead of storing them (forward and backward each run the filter).  Images are
+// FP32, scaled by 255 * input_scale before filtering, so an image u8/255 gives the maps of
+// the uint8 path up to the pyramid's rounding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/bimp_hip.h"
+#include "bank.hpp"
+#include "kernels.hpp"
+
+namespace bimpk {
+hipError_t diff_prepare(int n);
+hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, hipStream_t s);
+hipError_t diff_transpose(const float2* src, float2* dst, int rows, int cols, int batch,
+                          hipStream_t s);
+hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
+                         hipStream_t s);
+hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int sw, int sh,
+                             hipStream_t s);
+hipError_t diff_pad(const float* L, int H, int W, int FH, int FW, int k, float sc, float2* out,
+                    hipStream_t s);
+hipError_t diff_pad_adj(const float2* g, int H, int W, int FH, int FW, int k, float sc,
+                        float* gL, hipStream_t s);
+hipError_t diff_bank_pad(const float* ke, const float* ko, int kW, int kX, int FH, int FW,
+                         float2* out, hipStream_t s);
+hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, float invN,
+                         hipStream_t s);
+hipError_t diff_mag(const float2* Z, float* C, size_t n, hipStream_t s);
+hipError_t diff_cells_adj(const float* C, int FH, int FW, int H, int W, const CellParams& p,
+                          const float* gD, float* Wm, float* gC, hipStream_t s);
+hipError_t diff_grad_z(const float* gC, float2* Z, size_t n, hipStream_t s);
+hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_t n, float invN,
+                            hipStream_t s);
+hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, float invN, hipStream_t s);
+int diff_param_blocks(int kW);
+hipError_t diff_param_partials(const float2* Y, int kW, int kX, int FH, int FW, const float* dK,
+                               float* partials, hipStream_t s);
+hipError_t diff_fill(float* p, size_t n, hipStream_t s);
+}  // namespace bimpk
+
+namespace {
+
+using bimpk::kNumOri;
+
+struct DScale {
+    bimph::ScaleGeom g;
+    bimpk::CellParams cp;
+    float2* Q = nullptr;   // 8 x FH x FW bank spectra
+    float* dK = nullptr;   // 4 x 8 x kW x kW tap derivatives (dKe/dr, dKo/dr, dKe/dg, dKo/dg)
+    size_t respOff = 0;    // offset of this scale's D map in a response vector
+};
+
+int round_half_away(double v) { return (int)std::round(v); }
+
+}  // namespace
+
+struct bimp_diff {
+    int width = 0, height = 0;
+    bimp_options opt{};
+    float ratio = 0.f, gamma = 0.f;
+    std::vector<DScale> sc;
+    size_t respSize = 0;
+    int maxLevel = 0;
+    std::vector<int> lw, lh;
+    std::vector<float*> lev, glev;
+    std::map<int, float2*> tw;
+    size_t Pmax = 0;
+    float2 *Dhat = nullptr, *Z = nullptr, *T = nullptr, *acc = nullptr;
+    float *C = nullptr, *Wm = nullptr, *gC = nullptr, *S = nullptr, *taps = nullptr;
+    unsigned* maxslot = nullptr;
+    float* partials = nullptr;
+    int maxBlocks = 0;
+    std::string err;
+};
+
+namespace {
+
+int dfail(bimp_diff* d, int code, const std::string& m) {
+    if (d) d->err = m;
+    return code;
+}
+
+#define DCHK(d, expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return dfail(d, BIMP_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// In-place 2D complex transform of `nb` FH x FW maps (scratch of the same size).
+int fft2(bimp_diff* d, float2* a, int nb, int FH, int FW, bool inv, hipStream_t s) {
+    DCHK(d, bimpk::diff_rows(a, nb * FH, FW, inv, d->tw[FW], s));
+    DCHK(d, bimpk::diff_transpose(a, d->T, FH, FW, nb, s));
+    DCHK(d, bimpk::diff_rows(d->T, nb * FW, FH, inv, d->tw[FH], s));
+    DCHK(d, bimpk::diff_transpose(d->T, a, FW, FH, nb, s));
+    return BIMP_OK;
+}
+
+int set_params(bimp_diff* d, float ratio, float gamma, hipStream_t s) {
+    for (auto& x : d->sc) {
+        const auto& g = x.g;
+        const int kk = g.kW * g.kW;
+        std::vector<float> ke((size_t)kNumOri * kk), ko((size_t)kNumOri * kk);
+        std::vector<float> dk((size_t)4 * kNumOri * kk);
+        std::vector<double> a(kk), b(kk), c(kk), e(kk);
+        for (int o = 0; o < kNumOri; ++o) {
+            bimph::gabor_taps(g, o, gamma, ratio, ke.data() + (size_t)o * kk, ko.data() + (size_t)o * kk);
+            bimph::gabor_taps_grad(g, o, gamma, ratio, a.data(), b.data(), c.data(), e.data());
+            const size_t n = (size_t)kNumOri * kk;
+            for (int m = 0; m < kk; ++m) {
+                dk[0 * n + (size_t)o * kk + m] = (float)a[m];
+                dk[1 * n + (size_t)o * kk + m] = (float)b[m];
+                dk[2 * n + (size_t)o * kk + m] = (float)c[m];
+                dk[3 * n + (size_t)o * kk + m] = (float)e[m];
+            }
+        }
+        DCHK(d, hipMemcpy(d->taps, ke.data(), ke.size() * 4, hipMemcpyHostToDevice));
+        DCHK(d, hipMemcpy(d->taps + ke.size(), ko.data(), ko.size() * 4, hipMemcpyHostToDevice));
+        DCHK(d, hipMemcpy(x.dK, dk.data(), dk.size() * 4, hipMemcpyHostToDevice));
+        DCHK(d, bimpk::diff_bank_pad(d->taps, d->taps + ke.size(), g.kW, g.kX, g.FH, g.FW, x.Q, s));
+        const int st = fft2(d, x.Q, kNumOri, g.FH, g.FW, false, s);
+        if (st) return st;
+        DCHK(d, hipStreamSynchronize(s));
+    }
+    d->ratio = ratio;
+    d->gamma = gamma;
+    return BIMP_OK;
+}
+
+// Float pyramid of one image into d->lev.
+int pyramid(bimp_diff* d, const float* img, hipStream_t s) {
+    DCHK(d, hipMemcpyAsync(d->lev[0], img, (size_t)d->width * d->height * 4,
+                           hipMemcpyDeviceToDevice, s));
+    for (int l = 1; l <= d->maxLevel; ++l)
+        DCHK(d, bimpk::diff_pyr_down(d->lev[l - 1], d->lw[l - 1], d->lh[l - 1], d->lev[l], d->lw[l],
+                                     d->lh[l], s));
+    return BIMP_OK;
+}
+
+// Forward of one scale: Dhat, Z, C, and the D map into `resp`.
+int scale_forward(bimp_diff* d, DScale& x, float* resp, hipStream_t s) {
+    const auto& g = x.g;
+    const size_t P = (size_t)g.FH * g.FW;
+    const float sc = 255.0f * d->opt.input_scale;
+    DCHK(d, bimpk::diff_pad(d->lev[g.level], g.H, g.W, g.FH, g.FW, g.kX, sc, d->Dhat, s));
+    int st = fft2(d, d->Dhat, 1, g.FH, g.FW, false, s);
+    if (st) return st;
+    DCHK(d, bimpk::diff_spec_mul(d->Dhat, x.Q, d->Z, P, 1.0f / (float)P, s));
+    if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, true, s))) return st;
+    DCHK(d, bimpk::diff_mag(d->Z, d->C, (size_t)kNumOri * P, s));
+    DCHK(d, bimpk::launch_cells(d->C, g.FH, g.FW, g.FH, g.FW, g.H, g.W, x.cp, resp, d->S,
+                                d->maxslot, s));
+    return BIMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bimp_diff_create(int width, int height, const float* lambdas, int n, const bimp_options* opt,
+                     bimp_diff** out) {
+    if (!out) return BIMP_E_ARG;
+    *out = nullptr;
+    if (width < 8 || height < 8 || !lambdas || n <= 0 || n > 64) return BIMP_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BIMP_E_DEVICE;
+    bimp_diff* d = new bimp_diff;
+    d->width = width;
+    d->height = height;
+    if (opt)
+        d->opt = *opt;
+    else
+        bimp_default_options(&d->opt);
+    auto bail = [&](int code) {
+        bimp_diff_destroy(d);
+        return code;
+    };
+    std::vector<float> lam(lambdas, lambdas + n);
+    std::sort(lam.begin(), lam.end());
+    std::vector<int> sizes;
+    for (float l : lam) {
+        DScale x;
+        if (!bimph::derive_scale(l, width, height, d->opt.pyr_threshold, d->opt.support, &x.g) ||
+            x.g.lambda < 1.f || x.g.H < 4 || x.g.W < 4)
+            return bail(BIMP_E_ARG);
+        if (!bimpk::fft_size_supported(x.g.FH) || !bimpk::fft_size_supported(x.g.FW) ||
+            x.g.kW > x.g.FH || x.g.kW > x.g.FW)
+            return bail(BIMP_E_UNSUPPORTED);
+        for (int o = 0; o < kNumOri; ++o) {
+            const double theta = (double)(float)(o * 3.1415926535897932384626433832795 / kNumOri);
+            x.cp.ea[o] = round_half_away(0.6 * x.g.lambda * std::cos(theta));
+            x.cp.eb[o] = round_half_away(0.6 * x.g.lambda * std::sin(theta));
+            x.cp.ia[o] = round_half_away(0.6 * x.g.lambda * std::sin(theta));
+            x.cp.ib[o] = round_half_away(0.6 * x.g.lambda * std::cos(theta));
+            x.cp.ra[o] = round_half_away(0.5 * x.cp.ia[o]);
+            x.cp.rb[o] = round_half_away(0.5 * x.cp.ib[o]);
+        }
+        x.respOff = d->respSize;
+        d->respSize += (size_t)x.g.H * x.g.W;
+        d->maxLevel = std::max(d->maxLevel, x.g.level);
+        d->Pmax = std::max(d->Pmax, (size_t)x.g.FH * x.g.FW);
+        sizes.push_back(x.g.FH);
+        sizes.push_back(x.g.FW);
+        d->sc.push_back(x);
+    }
+    d->lw.assign(d->maxLevel + 1, 0);
+    d->lh.assign(d->maxLevel + 1, 0);
+    d->lw[0] = width;
+    d->lh[0] = height;
+    for (int l = 1; l <= d->maxLevel; ++l) {
+        d->lw[l] = (d->lw[l - 1] + 1) / 2;
+        d->lh[l] = (d->lh[l - 1] + 1) / 2;
+    }
+    d->lev.assign(d->maxLevel + 1, nullptr);
+    d->glev.assign(d->maxLevel + 1, nullptr);
+    for (int l = 0; l <= d->maxLevel; ++l)
+        if (hipMalloc(&d->lev[l], (size_t)d->lw[l] * d->lh[l] * 4) != hipSuccess ||
+            hipMalloc(&d->glev[l], (size_t)d->lw[l] * d->lh[l] * 4) != hipSuccess)
+            return bail(BIMP_E_ALLOC);
+    const size_t P = d->Pmax;
+    size_t maxTaps = 0;
+    for (auto& x : d->sc) {
+        const size_t kk = (size_t)x.g.kW * x.g.kW;
+        maxTaps = std::max(maxTaps, kk);
+        d->maxBlocks = std::max(d->maxBlocks, bimpk::diff_param_blocks(x.g.kW));
+        if (hipMalloc(&x.Q, (size_t)kNumOri * x.g.FH * x.g.FW * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&x.dK, 4 * kNumOri * kk * sizeof(float)) != hipSuccess)
+            return bail(BIMP_E_ALLOC);
+    }
+    if (hipMalloc(&d->Dhat, P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->acc, P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->Z, kNumOri * P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->T, kNumOri * P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->C, kNumOri * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->gC, kNumOri * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->Wm, 4 * kNumOri * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->S, 2 * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->taps, 2 * kNumOri * maxTaps * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->maxslot, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&d->partials, 2 * (size_t)d->maxBlocks * d->sc.size() * sizeof(float)) != hipSuccess)
+        return bail(BIMP_E_ALLOC);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    for (int n2 : sizes) {
+        if (bimpk::diff_prepare(n2) != hipSuccess || bimpk::prepare_size(n2) != hipSuccess)
+            return bail(BIMP_E_DEVICE);
+        std::vector<float2> t;
+        if (!bimpk::twiddle_table(n2, t)) return bail(BIMP_E_UNSUPPORTED);
+        float2* p = nullptr;
+        if (hipMalloc(&p, t.size() * sizeof(float2) + 256) != hipSuccess) return bail(BIMP_E_ALLOC);
+        d->tw[n2] = p;
+        if (hipMemcpy(p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(BIMP_E_DEVICE);
+    }
+    if (bimpk::prepare_common() != hipSuccess) return bail(BIMP_E_DEVICE);
+    const int st = set_params(d, d->opt.sigma_ratio, d->opt.gamma, nullptr);
+    if (st) return bail(st);
+    *out = d;
+    return BIMP_OK;
+}
+
+int bimp_diff_destroy(bimp_diff* d) {
+    if (!d) return BIMP_E_ARG;
+    (void)hipDeviceSynchronize();
+    for (auto& x : d->sc) {
+        (void)hipFree(x.Q);
+        (void)hipFree(x.dK);
+    }
+    for (auto* p : d->lev) (void)hipFree(p);
+    for (auto* p : d->glev) (void)hipFree(p);
+    for (auto& kv : d->tw) (void)hipFree(kv.second);
+    for (void* p : {(void*)d->Dhat, (void*)d->acc, (void*)d->Z, (void*)d->T, (void*)d->C,
+                    (void*)d->gC, (void*)d->Wm, (void*)d->S, (void*)d->taps, (void*)d->maxslot,
+                    (void*)d->partials})
+        (void)hipFree(p);
+    delete d;
+    return BIMP_OK;
+}
+
+long long bimp_diff_response_size(const bimp_diff* d) { return d ? (long long)d->respSize : BIMP_E_ARG; }
+
+int bimp_diff_set_params(bimp_diff* d, float sigma_ratio, float gamma) {
+    if (!d || !(sigma_ratio > 0.f) || !(gamma > 0.f)) return BIMP_E_ARG;
+    if (sigma_ratio == d->ratio && gamma == d->gamma) return BIMP_OK;
+    return set_params(d, sigma_ratio, gamma, nullptr);
+}
+
+int bimp_diff_forward(bimp_diff* d, const float* d_img, int nimg, float* d_resp,
+                      bimp_stream_t stream) {
+    if (!d || !d_img || !d_resp || nimg <= 0) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t isz = (size_t)d->width * d->height;
+    for (int b = 0; b < nimg; ++b) {
+        int st = pyramid(d, d_img + b * isz, s);
+        if (st) return st;
+        for (auto& x : d->sc)
+            if ((st = scale_forward(d, x, d_resp + b * d->respSize + x.respOff, s))) return st;
+    }
+    return BIMP_OK;
+}
+
+int bimp_diff_backward(bimp_diff* d, const float* d_img, int nimg, const float* d_grad_resp,
+                       float* d_grad_img, double* grad_params, bimp_stream_t stream) {
+    if (!d || !d_img || !d_grad_resp || !d_grad_img || nimg <= 0) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t isz = (size_t)d->width * d->height;
+    const float sc = 255.0f * d->opt.input_scale;
+    double gr = 0, gg = 0;
+    std::vector<float> part;
+    for (int b = 0; b < nimg; ++b) {
+        int st = pyramid(d, d_img + b * isz, s);
+        if (st) return st;
+        for (int l = 0; l <= d->maxLevel; ++l)
+            DCHK(d, bimpk::diff_fill(d->glev[l], (size_t)d->lw[l] * d->lh[l], s));
+        for (size_t si = 0; si < d->sc.size(); ++si) {
+            DScale& x = d->sc[si];
+            const auto& g = x.g;
+            const size_t P = (size_t)g.FH * g.FW;
+            const float invN = 1.0f / (float)P;
+            // recompute the responses (the D map goes to scratch)
+            if ((st = scale_forward(d, x, d->S + P, s))) return st;
+            DCHK(d, bimpk::diff_cells_adj(d->C, g.FH, g.FW, g.H, g.W, x.cp,
+                                          d_grad_resp + b * d->respSize + x.respOff, d->Wm, d->gC, s));
+            DCHK(d, bimpk::diff_grad_z(d->gC, d->Z, (size_t)kNumOri * P, s));
+            if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, false, s))) return st;
+            DCHK(d, bimpk::diff_accum_conjq(x.Q, d->Z, d->acc, P, invN, s));
+            if ((st = fft2(d, d->acc, 1, g.FH, g.FW, true, s))) return st;
+            DCHK(d, bimpk::diff_pad_adj(d->acc, g.H, g.W, g.FH, g.FW, g.kX, sc, d->glev[g.level], s));
+            DCHK(d, bimpk::diff_mul_conjd(d->Z, d->Dhat, P, invN, s));
+            if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, true, s))) return st;
+            DCHK(d, bimpk::diff_param_partials(d->Z, g.kW, g.kX, g.FH, g.FW, x.dK,
+                                               d->partials + 2 * (size_t)d->maxBlocks * si, s));
+        }
+        for (int l = d->maxLevel; l >= 1; --l)
+            DCHK(d, bimpk::diff_pyr_down_adj(d->glev[l], d->lw[l], d->lh[l], d->glev[l - 1],
+                                             d->lw[l - 1], d->lh[l - 1], s));
+        DCHK(d, hipMemcpyAsync(d_grad_img + b * isz, d->glev[0], isz * 4, hipMemcpyDeviceToDevice, s));
+        part.resize(2 * (size_t)d->maxBlocks * d->sc.size());
+        DCHK(d, hipMemcpyAsync(part.data(), d->partials, part.size() * 4, hipMemcpyDeviceToHost, s));
+        DCHK(d, hipStreamSynchronize(s));
+        for (size_t si = 0; si < d->sc.size(); ++si) {
+            const int nbk = bimpk::diff_param_blocks(d->sc[si].g.kW);
+            for (int k = 0; k < nbk; ++k) {
+                gr += part[2 * ((size_t)d->maxBlocks * si + k)];
+                gg += part[2 * ((size_t)d->maxBlocks * si + k) + 1];
+            }
+        }
+    }
+    if (grad_params) {
+        grad_params[0] = gr;
+        grad_params[1] = gg;
+    }
+    return BIMP_OK;
+}
+
+const char* bimp_diff_last_error(const bimp_diff* d) { return d ? d->err.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,498 @@
+// diff.hip -- differentiation form: the filter / complex-cell / end-stop / inhibition chain
+// on float images, and its hand-written adjoint.
+//
+// Forward (per scale, per image; FP32): level image L (float pyramid), padded image
+//   Pd = s * pad(L)                                 (clamp padding, convolutionFFT2D.cuh:56-85)
+//   Z_o = IFFT(FFT(Pd) * Qhat_o) / N,  Qhat_o = FFT(pad(Ke_o + i*Ko_o))   (cuda_bimp.cu:486-507)
+//   C_o = |Z_o|,  D = end-stop - inhibition          (cuda_bimp.cuh:436-517)
+// Adjoint, for an upstream gradient gD of D:
+//   gC_o from the stencil masks (two deterministic passes, no atomics),
+//   gZ_o = gC_o * Z_o / |Z_o|,  Ghat_o = FFT(gZ_o),
+//   gPd  = Re IFFT(sum_o conj(Qhat_o) * Ghat_o) / N,
+//   Y_o  = IFFT(Ghat_o * conj(FFT(Pd))) / N = dL/dKe_o + i dL/dKo_o at the tap positions,
+//   then gL = s * pad^T(gPd) and the float-pyramid adjoint down to the image.
+// All transforms are full complex 2D FFTs built from batched row transforms and tiled
+// transposes.
+#include <hip/hip_runtime.h>
+
+#include "fft_device.hpp"
+#include "kernels.hpp"
+
+namespace bimpk {
+
+namespace {
+
+using bimpdev::FFTCfg;
+using bimpdev::in_index;
+using bimpdev::out_index;
+
+constexpr int kRowThreads = 256;
+
+// Batched in-place complex transforms of `rows` contiguous rows of length N.
+template <int N, bool INV>
+__global__ void __launch_bounds__(kRowThreads) k_c2c_rows(float2* __restrict__ data, int rows,
+                                                          const float2* __restrict__ tw) {
+    using C = FFTCfg<N>;
+    constexpr int GPB = kRowThreads / C::TPV;
+    extern __shared__ float lds[];
+    const int grp = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
+    const int row = blockIdx.x * GPB + grp;
+    if (row >= rows) return;  // whole groups only: a group never spans two wavefronts
+    float2* p = data + (size_t)row * N;
+    float2 v[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) v[e] = p[in_index<N>(e, lane)];
+    bimpdev::fft<N, INV>(v, lds + grp * N, lane, tw);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) p[out_index<N>(e, lane)] = v[e];
+}
+
+// dst[b][c][r] = src[b][r][c] for rows x cols matrices, 32 x 32 tiles through LDS.
+__global__ void __launch_bounds__(256) k_transpose(const float2* __restrict__ src,
+                                                   float2* __restrict__ dst, int rows, int cols) {
+    __shared__ float2 t[32][33];
+    const size_t boff = (size_t)blockIdx.z * rows * cols;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) t[k][tx] = src[boff + (size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[boff + (size_t)c * rows + r] = t[tx][k];
+    }
+}
+
+__device__ __forceinline__ int reflect101(int t, int n) {
+    if (t < 0) t = -t;
+    if (t >= n) t = 2 * (n - 1) - t;
+    return t;
+}
+
+// Float pyramid level (the 5x5 binomial / 256 of pyrDown with reflect-101 borders, no rounding).
+__global__ void k_pyr_down_f(const float* __restrict__ src, int sw, int sh,
+                             float* __restrict__ dst, int dw, int dh) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= dw) return;
+    const float k[5] = {1.f, 4.f, 6.f, 4.f, 1.f};
+    float acc = 0.f;
+    for (int i = -2; i <= 2; ++i) {
+        const float* row = src + (size_t)reflect101(2 * y + i, sh) * sw;
+        float r = 0.f;
+        for (int j = -2; j <= 2; ++j) r += k[j + 2] * row[reflect101(2 * x + j, sw)];
+        acc += k[i + 2] * r;
+    }
+    dst[(size_t)y * dw + x] = acc * (1.f / 256.f);
+}
+
+// Weights of the pyramid's 1D stencil seen from source index p: every (d, w) with
+// src index p receiving weight w/16 from destination d.  Preimages of p under reflect-101
+// are p, -p (p > 0) and 2(n-1) - p (p < n-1).
+__device__ __forceinline__ int pyr_taps(int p, int n, int dn, int* d, float* w) {
+    const float k[5] = {1.f, 4.f, 6.f, 4.f, 1.f};
+    int cnt = 0;
+    int pre[3], np = 0;
+    pre[np++] = p;
+    if (p > 0) pre[np++] = -p;
+    if (p < n - 1) pre[np++] = 2 * (n - 1) - p;
+    for (int a = 0; a < np; ++a) {
+        const int t = pre[a];
+        for (int i = -2; i <= 2; ++i) {
+            const int q = t - i;
+            if (q & 1) continue;
+            const int dd = q / 2;
+            if (dd < 0 || dd >= dn) continue;
+            d[cnt] = dd;
+            w[cnt] = k[i + 2] * (1.f / 16.f);
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+// gsrc += pyrDown^T(gdst)
+__global__ void k_pyr_down_adj(const float* __restrict__ gdst, int dw, int dh,
+                               float* __restrict__ gsrc, int sw, int sh) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= sw) return;
+    int dy[9], dx[9];
+    float wy[9], wx[9];
+    const int ny = pyr_taps(y, sh, dh, dy, wy);
+    const int nx = pyr_taps(x, sw, dw, dx, wx);
+    float acc = 0.f;
+    for (int a = 0; a < ny; ++a) {
+        float r = 0.f;
+        for (int b = 0; b < nx; ++b) r += wx[b] * gdst[(size_t)dy[a] * dw + dx[b]];
+        acc += wy[a] * r;
+    }
+    gsrc[(size_t)y * sw + x] += acc;
+}
+
+__device__ __forceinline__ int clamp_src(int y, int H, int k) { return y < H ? y : (y < H + k ? H - 1 : 0); }
+
+// Pd = s * pad(L) as complex (imaginary part 0).
+__global__ void k_pad_fwd(const float* __restrict__ L, int H, int W, int FH, int FW, int k,
+                          float s, float2* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= FW) return;
+    out[(size_t)y * FW + x] = make_float2(s * L[(size_t)clamp_src(y, H, k) * W + clamp_src(x, W, k)], 0.f);
+}
+
+// Source rows of the padded image that read data row r: r itself, the band [H, H+k) for
+// r = H-1 and the band [H+k, F) for r = 0.
+__device__ __forceinline__ void pad_band(int r, int H, int F, int k, int* lo1, int* hi1) {
+    *lo1 = 0;
+    *hi1 = 0;
+    if (r == H - 1) {
+        *lo1 = H;
+        *hi1 = min(H + k, F);
+    } else if (r == 0) {
+        *lo1 = min(H + k, F);
+        *hi1 = F;
+    }
+}
+
+// gL += s * pad^T(Re gPd)
+__global__ void k_pad_adj(const float2* __restrict__ g, int H, int W, int FH, int FW, int k,
+                          float s, float* __restrict__ gL) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    int ylo, yhi, xlo, xhi;
+    pad_band(y, H, FH, k, &ylo, &yhi);
+    pad_band(x, W, FW, k, &xlo, &xhi);
+    float acc = 0.f;
+    for (int yy = -1; yy < yhi - ylo; ++yy) {
+        const int sy = yy < 0 ? y : ylo + yy;
+        const float2* row = g + (size_t)sy * FW;
+        acc += row[x].x;
+        for (int xx = xlo; xx < xhi; ++xx) acc += row[xx].x;
+    }
+    gL[(size_t)y * W + x] += s * acc;
+}
+
+// Padded complex kernels of the 8 orientations: tap (r, c) sits at ((r-kX) mod FH,
+// (c-kX) mod FW) (requires kW <= FH, FW, so positions are distinct).
+__global__ void k_bank_pad(const float* __restrict__ ke, const float* __restrict__ ko, int kW,
+                           int kX, int FH, int FW, float2* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, o = blockIdx.z;
+    if (x >= FW) return;
+    const int r = (y + kX) % FH, c = (x + kX) % FW;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < kW && c < kW) {
+        const size_t m = ((size_t)o * kW + r) * kW + c;
+        v = make_float2(ke[m], ko[m]);
+    }
+    out[((size_t)o * FH + y) * FW + x] = v;
+}
+
+// Z_o = D * Q_o * invN for the 8 orientations.
+__global__ void k_spec_mul(const float2* __restrict__ D, const float2* __restrict__ Q,
+                           float2* __restrict__ Z, size_t n, float invN) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 d = D[i];
+#pragma unroll
+    for (int o = 0; o < kNumOri; ++o) {
+        const float2 q = Q[(size_t)o * n + i];
+        Z[(size_t)o * n + i] =
+            make_float2((d.x * q.x - d.y * q.y) * invN, (d.x * q.y + d.y * q.x) * invN);
+    }
+}
+
+__global__ void k_mag(const float2* __restrict__ Z, float* __restrict__ C, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 z = Z[i];
+    C[i] = sqrtf(z.x * z.x + z.y * z.y);
+}
+
+// Complex-cell map read with the reference's linear index y*FW + x over the FH x FW array,
+// wrapping past the end (the forward stencil's semantics).
+__device__ __forceinline__ float cat(const float* C, int o, long idx, long P) {
+    if (idx >= P) idx -= P;
+    if (idx < 0) idx += P;
+    return C[(size_t)o * P + idx];
+}
+
+__device__ __forceinline__ bool valid(int x, int y, int a, int b, int W, int H) {
+    const int aa = a < 0 ? -a : a, bb = b < 0 ? -b : b;
+    return 2 * bb <= x && x <= W - 2 * bb && 2 * aa <= y && y <= H - 2 * aa;
+}
+
+// Pass 1 of the stencil adjoint: per pixel and orientation, upstream gradient times the
+// active-branch masks of the forward (cuda_bimp.cuh:436-502):
+//   Wd = g [ds > 0], Wt = g [tangential > 0], W1 = g [v2 > 0][v1 > 0], W2 = g [v2 > 0]
+// stored as 4 x 8 maps over the FH x FW grid (zero outside the data region).
+__global__ void k_cells_adj_w(const float* __restrict__ C, int FH, int FW, int H, int W,
+                              CellParams p, const float* __restrict__ gD, float* __restrict__ Wm) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= FW) return;
+    const long P = (long)FH * FW, g0 = (long)y * FW + x;
+    const float g = (y < H && x < W) ? gD[(size_t)y * W + x] : 0.f;
+#pragma unroll
+    for (int o = 0; o < kNumOri; ++o) {
+        float wd = 0.f, wt = 0.f, w1 = 0.f, w2 = 0.f;
+        if (g != 0.f) {
+            const int a = p.ea[o], b = p.eb[o];
+            if (valid(x, y, a, b, W, H)) {
+                const float v = (float)((double)cat(C, o, g0, P) -
+                                        0.5 * (double)cat(C, o, g0 - 2L * a * FW + 2 * b, P) -
+                                        0.5 * (double)cat(C, o, g0 + 2L * a * FW - 2 * b, P));
+                if (v > 0.f) wd = g;
+            }
+            const int ia = p.ia[o], ib = p.ib[o];
+            if (valid(x, y, ia, ib, W, H)) {
+                const int o2 = (o + kNumOri / 2) % kNumOri;
+                const float c = cat(C, o, g0, P);
+                const float vt = cat(C, o, g0 - (long)ia * FW - ib, P) - c;
+                if (vt > 0.f) wt = g;
+                const long rr = (long)p.ra[o] * FW + p.rb[o];
+                float v1 = c - 16.f * cat(C, o2, g0 + rr, P);
+                const bool m1 = v1 > 0.f;
+                v1 = m1 ? v1 : 0.f;
+                const float v2 = v1 + c - 16.f * cat(C, o2, g0 - rr, P);
+                if (v2 > 0.f) {
+                    w2 = g;
+                    w1 = m1 ? g : 0.f;
+                }
+            }
+        }
+        Wm[((size_t)0 * kNumOri + o) * P + g0] = wd;
+        Wm[((size_t)1 * kNumOri + o) * P + g0] = wt;
+        Wm[((size_t)2 * kNumOri + o) * P + g0] = w1;
+        Wm[((size_t)3 * kNumOri + o) * P + g0] = w2;
+    }
+}
+
+// Pass 2: gather, for each map position p and orientation o, every forward read of C_o(p):
+//   D = sum_o relu(C(x) - C(x+d1)/2 - C(x+d2)/2) - sum_o [relu(C(x+t) - C(x)) + R_o(x)],
+//   R_o(x) = relu(relu(C_o(x) - 16 C_o'(x+r)) + C_o(x) - 16 C_o'(x-r)), o' = o+4.
+__global__ void k_cells_adj_g(const float* __restrict__ Wm, int FH, int FW, CellParams p,
+                              float* __restrict__ gC) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, o = blockIdx.z;
+    if (x >= FW) return;
+    const long P = (long)FH * FW, q = (long)y * FW + x;
+    const float* Wd = Wm + ((size_t)0 * kNumOri + o) * P;
+    const float* Wt = Wm + ((size_t)1 * kNumOri + o) * P;
+    const int a = p.ea[o], b = p.eb[o];
+    const long d1 = -2L * a * FW + 2 * b;
+    float acc = cat(Wd, 0, q, P) - 0.5f * cat(Wd, 0, q - d1, P) - 0.5f * cat(Wd, 0, q + d1, P);
+    const long t = -(long)p.ia[o] * FW - p.ib[o];
+    acc += cat(Wt, 0, q, P) - cat(Wt, 0, q - t, P);
+    // own radial term (centre)
+    const float* W1 = Wm + ((size_t)2 * kNumOri + o) * P;
+    const float* W2 = Wm + ((size_t)3 * kNumOri + o) * P;
+    acc -= cat(W1, 0, q, P) + cat(W2, 0, q, P);
+    // partner terms: C_o is the second map of orientation o' = o + 4 (mod 8)
+    const int op = (o + kNumOri / 2) % kNumOri;
+    const float* W1p = Wm + ((size_t)2 * kNumOri + op) * P;
+    const float* W2p = Wm + ((size_t)3 * kNumOri + op) * P;
+    const long rr = (long)p.ra[op] * FW + p.rb[op];
+    acc += 16.f * cat(W1p, 0, q - rr, P) + 16.f * cat(W2p, 0, q + rr, P);
+    gC[(size_t)o * P + q] = acc;
+}
+
+// gZ = gC * Z / |Z| in place over Z (zero where |Z| = 0).
+__global__ void k_grad_z(const float* __restrict__ gC, float2* __restrict__ Z, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 z = Z[i];
+    const float m = sqrtf(z.x * z.x + z.y * z.y);
+    const float s = m > 0.f ? gC[i] / m : 0.f;
+    Z[i] = make_float2(z.x * s, z.y * s);
+}
+
+// out = invN * sum_o conj(Q_o) * G_o
+__global__ void k_accum_conjq(const float2* __restrict__ Q, const float2* __restrict__ G,
+                              float2* __restrict__ out, size_t n, float invN) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int o = 0; o < kNumOri; ++o) {
+        const float2 q = Q[(size_t)o * n + i], g = G[(size_t)o * n + i];
+        re += q.x * g.x + q.y * g.y;
+        im += q.x * g.y - q.y * g.x;
+    }
+    out[i] = make_float2(re * invN, im * invN);
+}
+
+// G_o = invN * G_o * conj(D) in place.
+__global__ void k_mul_conjd(float2* __restrict__ G, const float2* __restrict__ D, size_t n,
+                            float invN) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 d = D[i];
+#pragma unroll
+    for (int o = 0; o < kNumOri; ++o) {
+        const float2 g = G[(size_t)o * n + i];
+        G[(size_t)o * n + i] =
+            make_float2((g.x * d.x + g.y * d.y) * invN, (g.y * d.x - g.x * d.y) * invN);
+    }
+}
+
+// Parameter gradient partials: sum over orientations and taps of
+//   Re Y_o(tap) dKe/dtheta + Im Y_o(tap) dKo/dtheta  for theta = (sigma ratio, gamma);
+// one pair of partial sums per block (fixed-order tree), summed on the host in block order.
+__global__ void __launch_bounds__(256) k_param_partials(const float2* __restrict__ Y, int kW,
+                                                        int kX, int FH, int FW,
+                                                        const float* __restrict__ dK,
+                                                        float* __restrict__ partials) {
+    __shared__ float sr[256], sg[256];
+    const int n = kNumOri * kW * kW;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float vr = 0.f, vg = 0.f;
+    if (i < n) {
+        const int o = i / (kW * kW), m = i % (kW * kW), r = m / kW, c = m % kW;
+        const int y = (r - kX + FH) % FH, x = (c - kX + FW) % FW;
+        const float2 v = Y[((size_t)o * FH + y) * FW + x];
+        const size_t stride = (size_t)n;
+        vr = v.x * dK[0 * stride + i] + v.y * dK[1 * stride + i];
+        vg = v.x * dK[2 * stride + i] + v.y * dK[3 * stride + i];
+    }
+    sr[threadIdx.x] = vr;
+    sg[threadIdx.x] = vg;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            sr[threadIdx.x] += sr[threadIdx.x + h];
+            sg[threadIdx.x] += sg[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sr[0];
+        partials[2 * blockIdx.x + 1] = sg[0];
+    }
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+inline dim3 ew(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+hipError_t diff_prepare(int n) {
+#define X(N)                                                                                 \
+    if (n == N) {                                                                            \
+        const int lds = (kRowThreads / FFTCfg<N>::TPV) * N * (int)sizeof(float);             \
+        hipError_t e = hipFuncSetAttribute((const void*)k_c2c_rows<N, false>,                \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+        if (e != hipSuccess) return e;                                                       \
+        return hipFuncSetAttribute((const void*)k_c2c_rows<N, true>,                         \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);         \
+    }
+    BIMP_FOR_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, hipStream_t s) {
+#define X(N)                                                                                 \
+    if (n == N) {                                                                            \
+        constexpr int GPB = kRowThreads / FFTCfg<N>::TPV;                                    \
+        const int lds = GPB * N * (int)sizeof(float);                                        \
+        dim3 grid((rows + GPB - 1) / GPB);                                                   \
+        if (inv)                                                                             \
+            k_c2c_rows<N, true><<<grid, kRowThreads, lds, s>>>(data, rows, tw);              \
+        else                                                                                 \
+            k_c2c_rows<N, false><<<grid, kRowThreads, lds, s>>>(data, rows, tw);             \
+        return hipGetLastError();                                                            \
+    }
+    BIMP_FOR_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t diff_transpose(const float2* src, float2* dst, int rows, int cols, int batch,
+                          hipStream_t s) {
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32, batch);
+    k_transpose<<<grid, 256, 0, s>>>(src, dst, rows, cols);
+    return hipGetLastError();
+}
+
+hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
+                         hipStream_t s) {
+    k_pyr_down_f<<<dim3((dw + 127) / 128, dh), 128, 0, s>>>(src, sw, sh, dst, dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int sw, int sh,
+                             hipStream_t s) {
+    k_pyr_down_adj<<<dim3((sw + 127) / 128, sh), 128, 0, s>>>(gdst, dw, dh, gsrc, sw, sh);
+    return hipGetLastError();
+}
+
+hipError_t diff_pad(const float* L, int H, int W, int FH, int FW, int k, float sc, float2* out,
+                    hipStream_t s) {
+    k_pad_fwd<<<dim3((FW + 127) / 128, FH), 128, 0, s>>>(L, H, W, FH, FW, k, sc, out);
+    return hipGetLastError();
+}
+
+hipError_t diff_pad_adj(const float2* g, int H, int W, int FH, int FW, int k, float sc,
+                        float* gL, hipStream_t s) {
+    k_pad_adj<<<dim3((W + 127) / 128, H), 128, 0, s>>>(g, H, W, FH, FW, k, sc, gL);
+    return hipGetLastError();
+}
+
+hipError_t diff_bank_pad(const float* ke, const float* ko, int kW, int kX, int FH, int FW,
+                         float2* out, hipStream_t s) {
+    k_bank_pad<<<dim3((FW + 127) / 128, FH, kNumOri), 128, 0, s>>>(ke, ko, kW, kX, FH, FW, out);
+    return hipGetLastError();
+}
+
+hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, float invN,
+                         hipStream_t s) {
+    k_spec_mul<<<ew(n), 256, 0, s>>>(D, Q, Z, n, invN);
+    return hipGetLastError();
+}
+
+hipError_t diff_mag(const float2* Z, float* C, size_t n, hipStream_t s) {
+    k_mag<<<ew(n), 256, 0, s>>>(Z, C, n);
+    return hipGetLastError();
+}
+
+hipError_t diff_cells_adj(const float* C, int FH, int FW, int H, int W, const CellParams& p,
+                          const float* gD, float* Wm, float* gC, hipStream_t s) {
+    k_cells_adj_w<<<dim3((FW + 127) / 128, FH), 128, 0, s>>>(C, FH, FW, H, W, p, gD, Wm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_cells_adj_g<<<dim3((FW + 127) / 128, FH, kNumOri), 128, 0, s>>>(Wm, FH, FW, p, gC);
+    return hipGetLastError();
+}
+
+hipError_t diff_grad_z(const float* gC, float2* Z, size_t n, hipStream_t s) {
+    k_grad_z<<<ew(n), 256, 0, s>>>(gC, Z, n);
+    return hipGetLastError();
+}
+
+hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_t n, float invN,
+                            hipStream_t s) {
+    k_accum_conjq<<<ew(n), 256, 0, s>>>(Q, G, out, n, invN);
+    return hipGetLastError();
+}
+
+hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, float invN, hipStream_t s) {
+    k_mul_conjd<<<ew(n), 256, 0, s>>>(G, D, n, invN);
+    return hipGetLastError();
+}
+
+int diff_param_blocks(int kW) { return (kNumOri * kW * kW + 255) / 256; }
+
+hipError_t diff_param_partials(const float2* Y, int kW, int kX, int FH, int FW, const float* dK,
+                               float* partials, hipStream_t s) {
+    k_param_partials<<<diff_param_blocks(kW), 256, 0, s>>>(Y, kW, kX, FH, FW, dK, partials);
+    return hipGetLastError();
+}
+
+hipError_t diff_fill(float* p, size_t n, hipStream_t s) {
+    k_fill<<<ew(n), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace bimpk

@@ -754,7 +754,6 @@ __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, s
 
 // ------------------------------------------------------------------ dispatch
 #define BIMP_CELL_HALOS(X) X(4) X(8) X(12) X(16) X(20)
-#define BIMP_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(4096)
 
 // Host construction of the packed twiddle table of FFTCfg<N> (layout: TwLayout).
 template <class L, int Ns>

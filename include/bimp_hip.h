@@ -242,6 +242,30 @@ int bimp_batch_time(bimp_batch* batch, int iters, bimp_stream_t stream, float* b
 int bimp_batch_plan(const bimp_batch* batch, char* buf, int cap);
 const char* bimp_batch_last_error(const bimp_batch* batch);
 
+/* ---- Differentiation form -------------------------------------------------------------
+ * The filter bank, complex cells, end-stopping and inhibition (cuda_bimp.cu:462-552; NMS
+ * excluded) as a differentiable map from float images to the per-scale response maps D, with
+ * a hand-written adjoint.  No counterpart in the reference.  Images are nimg x height x width
+ * FP32 (device, contiguous), scaled by 255 * input_scale before filtering (an image u8/255
+ * gives the uint8 path's maps up to the pyramid's rounding; levels above 0 use the float
+ * pyramid).  Responses are nimg x bimp_diff_response_size() floats: the D maps of the scales in
+ * ascending wavelength order, each data_h x data_w.  The Gabor parameters (sigma ratio, gamma)
+ * start at opt's values.  The backward pass recomputes the complex responses, writes
+ * d(loss)/d(image) for an upstream gradient of the responses, and the gradient with respect to
+ * (sigma ratio, gamma) summed over the images into grad_params (host, 2 doubles).  It
+ * synchronises the stream. */
+typedef struct bimp_diff bimp_diff;
+int bimp_diff_create(int width, int height, const float* lambdas, int n, const bimp_options* opt,
+                     bimp_diff** out);
+int bimp_diff_destroy(bimp_diff* d);
+long long bimp_diff_response_size(const bimp_diff* d);
+int bimp_diff_set_params(bimp_diff* d, float sigma_ratio, float gamma);
+int bimp_diff_forward(bimp_diff* d, const float* d_img, int nimg, float* d_resp,
+                      bimp_stream_t stream);
+int bimp_diff_backward(bimp_diff* d, const float* d_img, int nimg, const float* d_grad_resp,
+                       float* d_grad_img, double* grad_params, bimp_stream_t stream);
+const char* bimp_diff_last_error(const bimp_diff* d);
+
 #ifdef __cplusplus
 }
 #endif
