@@ -9,8 +9,9 @@ Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel, CPU 
 The same line carries the precision form (BF16 / FP16 direct convolution on the matrix cores
 for the scales with lambda <= 8, the others FFT-FP32): its frames/s, speed-up over the plain
 path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle,
-and the batching form (64 frames of five sizes as one tuned hipGraph launch): aggregate
-megapixels/s and per-frame latency.
+the batching form (64 frames of five sizes as one tuned hipGraph launch): aggregate
+megapixels/s and per-frame latency, and the differentiation form (16 FP32 512x512 images
+through the libtorch autograd Function): backward time per image and gradient checks.
 """
 import argparse
 import json
@@ -192,6 +193,89 @@ def batch_form(B, stream, frames=64, seed=3, iters=3):
     return out
 
 
+def diff_form(B, nimg=16, size=512, check_fp64=True):
+    """Differentiation form: 16 FP32 512x512 images (40 shapes, seed 2000 + i, / 255); loss =
+    sum of squared differences between the response maps and the responses of the same images
+    under gamma = 0.6; gradients for the images and (sigma ratio, gamma) by the adjoint."""
+    import numpy as np
+    import torch
+    from bimp_amd.autograd import BimpResponses
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from diff_ref import DiffRef
+    R = BimpResponses(size, size, bank())
+    imgs = np.stack([B.synth_frame(size, size, 40, 2000 + i) for i in range(nimg)]).astype(np.float32) / 255.0
+    x = torch.tensor(imgs, device="cuda")
+    p0 = torch.tensor([0.56, 0.5], dtype=torch.float64)
+    with torch.no_grad():
+        target = R(x, torch.tensor([0.56, 0.6], dtype=torch.float64))
+
+    def step():
+        xi = x.detach().requires_grad_(True)
+        p = p0.clone().requires_grad_(True)
+        resp = R(xi, p)
+        loss = ((resp - target) ** 2).sum()
+        loss.backward()
+        return float(loss), xi.grad, p.grad
+
+    step()  # warm-up
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd_ms, bwd_ms = [], []
+    for _ in range(3):
+        xi = x.detach().requires_grad_(True)
+        p = p0.clone().requires_grad_(True)
+        ev[0].record()
+        resp = R(xi, p)
+        loss = ((resp - target) ** 2).sum()
+        ev[1].record()
+        loss.backward()
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd_ms.append(ev[0].elapsed_time(ev[1]))
+        bwd_ms.append(ev[1].elapsed_time(ev[2]))
+    loss, gx, gp = step()
+    out = {"images": nimg, "size": [size, size], "scales": len(bank()),
+           "forward_ms_per_image": round(min(fwd_ms) / nimg, 4),
+           "backward_ms_per_image": round(min(bwd_ms) / nimg, 4),
+           "loss": loss, "grad_params": [float(v) for v in gp]}
+    # central differences (h = 1e-3) of the FP32 loss on image 0
+    h = 1e-3
+    x0 = x[:1]
+    t0 = target[:1]
+
+    def L(im, pv):
+        with torch.no_grad():
+            r = R(im, torch.tensor(pv, dtype=torch.float64))
+            return float(((r.double() - t0.double()) ** 2).sum())
+
+    g0x, g0p = None, None
+    xi = x0.detach().requires_grad_(True)
+    pp = p0.clone().requires_grad_(True)
+    ((R(xi, pp) - t0) ** 2).sum().backward()
+    g0x, g0p = xi.grad[0].cpu().numpy(), pp.grad.numpy()
+    u = torch.tensor(np.random.default_rng(0).standard_normal((1, size, size)).astype(np.float32), device="cuda")
+    fd_u = (L(x0 + h * u, [0.56, 0.5]) - L(x0 - h * u, [0.56, 0.5])) / (2 * h)
+    an_u = float((g0x * u[0].cpu().numpy()).sum())
+    fd = {"image_direction": {"fd": fd_u, "adjoint": an_u, "rel_err": abs(fd_u - an_u) / abs(an_u)}}
+    for k, name in enumerate(("sigma_ratio", "gamma")):
+        a, b = [0.56, 0.5], [0.56, 0.5]
+        a[k] += h
+        b[k] -= h
+        f = (L(x0, a) - L(x0, b)) / (2 * h)
+        fd[name] = {"fd": f, "adjoint": float(g0p[k]), "rel_err": abs(f - g0p[k]) / abs(g0p[k])}
+    out["central_differences_h1e-3"] = fd
+    if check_fp64:
+        ref = DiffRef(size, size, bank())
+        img0 = imgs[0].astype(np.float64)
+        tg = ref.forward(img0, gamma=0.6)
+        rl, rg, rth = ref.gradient(img0, tg)
+        out["fp64_reference"] = {
+            "image_grad_rel_l2": float(np.linalg.norm(g0x - rg) / np.linalg.norm(rg)),
+            "param_grad_rel": [float(abs(g0p[k] - rth[k]) / abs(rth[k])) for k in range(2)],
+            "loss_rel": abs(L(x0, [0.56, 0.5]) - rl) / rl}
+    R.close()
+    return out
+
+
 def load_traffic(kernel):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -212,6 +296,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the BF16/FP16 precision-form lines and the FP64 oracle check")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form line")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form line")
     ap.add_argument("--form", choices=["plain", "bf16", "fp16"], default="plain",
                     help="path timed for the main line (plain = FFT-FP32, the headline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -322,6 +407,10 @@ def main():
     if not args.no_batch and args.form == "plain":
         out["batch_form"] = batch_form(B, stream)
         print(f"batch form: {out['batch_form']['mpix_per_s']} MPix/s", file=sys.stderr, flush=True)
+    if not args.no_diff and args.form == "plain":
+        out["diff_form"] = diff_form(B)
+        print(f"diff form: backward {out['diff_form']['backward_ms_per_image']} ms/image",
+              file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_frames)
     else:

@@ -76,6 +76,8 @@ EXPORTED = [
     "bimp_batch_create", "bimp_batch_destroy", "bimp_batch_frame", "bimp_batch_run",
     "bimp_batch_results", "bimp_batch_outputs", "bimp_batch_time", "bimp_batch_plan",
     "bimp_batch_last_error",
+    "bimp_diff_create", "bimp_diff_destroy", "bimp_diff_response_size", "bimp_diff_set_params",
+    "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error",
 ]
 
 
@@ -127,6 +129,13 @@ def _load():
         "bimp_batch_time": (I, [P, I, P, C.POINTER(C.c_float), P]),
         "bimp_batch_plan": (I, [P, C.c_char_p, I]),
         "bimp_batch_last_error": (C.c_char_p, [P]),
+        "bimp_diff_create": (I, [I, I, P, I, C.POINTER(Options), C.POINTER(P)]),
+        "bimp_diff_destroy": (I, [P]),
+        "bimp_diff_response_size": (C.c_longlong, [P]),
+        "bimp_diff_set_params": (I, [P, C.c_float, C.c_float]),
+        "bimp_diff_forward": (I, [P, P, I, P, P]),
+        "bimp_diff_backward": (I, [P, P, I, P, P, P, P]),
+        "bimp_diff_last_error": (C.c_char_p, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -384,6 +393,51 @@ class Batch:
         if getattr(self, "b", None):
             lib.bimp_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        self.close()
+
+
+class Diff:
+    """Differentiation form (bimp_diff_*): float images -> per-scale D maps, and the adjoint.
+    Device buffers are passed as raw pointers (e.g. torch tensors' data_ptr())."""
+
+    def __init__(self, width, height, lambdas, options=None):
+        lam = np.ascontiguousarray(lambdas, np.float32)
+        opt = options if options is not None else default_options()
+        d = C.c_void_p()
+        st = lib.bimp_diff_create(int(width), int(height), _ptr(lam), lam.size, C.byref(opt),
+                                  C.byref(d))
+        if st < 0:
+            raise BimpError(st, lib.bimp_status_string(st).decode())
+        self.d = d
+        self.width, self.height = int(width), int(height)
+        self.response_size = int(lib.bimp_diff_response_size(d))
+
+    def _chk(self, st):
+        if st < 0:
+            raise BimpError(st, lib.bimp_diff_last_error(self.d).decode() or
+                            lib.bimp_status_string(st).decode())
+        return st
+
+    def set_params(self, sigma_ratio, gamma):
+        self._chk(lib.bimp_diff_set_params(self.d, float(sigma_ratio), float(gamma)))
+
+    def forward(self, img_ptr, nimg, resp_ptr, stream=None):
+        self._chk(lib.bimp_diff_forward(self.d, C.c_void_p(img_ptr), int(nimg),
+                                        C.c_void_p(resp_ptr), stream))
+
+    def backward(self, img_ptr, nimg, gresp_ptr, gimg_ptr, stream=None):
+        gp = np.zeros(2, np.float64)
+        self._chk(lib.bimp_diff_backward(self.d, C.c_void_p(img_ptr), int(nimg),
+                                         C.c_void_p(gresp_ptr), C.c_void_p(gimg_ptr), _ptr(gp),
+                                         stream))
+        return gp
+
+    def close(self):
+        if getattr(self, "d", None):
+            lib.bimp_diff_destroy(self.d)
+            self.d = None
 
     def __del__(self):
         self.close()
