@@ -252,10 +252,16 @@ def diff_form(B, nimg=16, size=512, check_fp64=True):
     pp = p0.clone().requires_grad_(True)
     ((R(xi, pp) - t0) ** 2).sum().backward()
     g0x, g0p = xi.grad[0].cpu().numpy(), pp.grad.numpy()
-    u = torch.tensor(np.random.default_rng(0).standard_normal((1, size, size)).astype(np.float32), device="cuda")
-    fd_u = (L(x0 + h * u, [0.56, 0.5]) - L(x0 - h * u, [0.56, 0.5])) / (2 * h)
-    an_u = float((g0x * u[0].cpu().numpy()).sum())
-    fd = {"image_direction": {"fd": fd_u, "adjoint": an_u, "rel_err": abs(fd_u - an_u) / abs(an_u)}}
+    yy, xx = np.mgrid[0:size, 0:size]
+    dirs = {"image_smooth_direction": np.sin(2 * np.pi * xx / size) * np.cos(2 * np.pi * yy / size),
+            "image_random_direction_kink_limited":
+                np.random.default_rng(0).standard_normal((size, size))}
+    fd = {}
+    for name, uu in dirs.items():
+        u = torch.tensor(uu[None].astype(np.float32), device="cuda")
+        fd_u = (L(x0 + h * u, [0.56, 0.5]) - L(x0 - h * u, [0.56, 0.5])) / (2 * h)
+        an_u = float((g0x * uu).sum())
+        fd[name] = {"fd": fd_u, "adjoint": an_u, "rel_err": abs(fd_u - an_u) / abs(an_u)}
     for k, name in enumerate(("sigma_ratio", "gamma")):
         a, b = [0.56, 0.5], [0.56, 0.5]
         a[k] += h

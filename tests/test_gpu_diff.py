@@ -2,8 +2,8 @@
 
 Bars: responses within 1e-4 of the FP64 restatement (tests/diff_ref.py); image and parameter
 gradients within 1e-3 (relative L2 / relative) of its FP64 reference gradient; central
-differences of the FP32 loss (h = 1e-3) agree with the adjoint's directional derivatives to
-the accuracy such a step allows (5e-2: the step crosses relu branches)."""
+differences of the FP32 loss (h = 1e-3) along a smooth image direction and along each
+parameter within 1e-2 of the adjoint's directional derivatives."""
 import numpy as np
 import pytest
 
@@ -94,19 +94,21 @@ def test_central_differences_h1e3(T, R):
             r = R(T.tensor(im, device="cuda"), T.tensor(p, dtype=T.float64))
             return float(((r.double() - tgt.double()) ** 2).sum())
 
-    rng = np.random.default_rng(1)
-    u = rng.standard_normal(img.shape).astype(np.float32)
+    # a smooth image direction: a per-pixel random one at this step flips relu branches of
+    # the stencil everywhere and measures the kinks, not the derivative (about 15% in FP64)
+    yy, xx = np.mgrid[0:H, 0:W]
+    u = (np.sin(2 * np.pi * xx / W) * np.cos(2 * np.pi * yy / H)).astype(np.float32)
     fd = (L((img + h * u)[None], [0.56, 0.5]) - L((img - h * u)[None], [0.56, 0.5])) / (2 * h)
     an = float((gx[0] * u).sum())
     print(f"image direction: fd {fd:.6g} adjoint {an:.6g}", flush=True)
-    assert abs(fd - an) <= 5e-2 * abs(an)
+    assert abs(fd - an) <= 1e-2 * abs(an)
     for k in range(2):
         pp, pm = [0.56, 0.5], [0.56, 0.5]
         pp[k] += h
         pm[k] -= h
         fdp = (L(img[None], pp) - L(img[None], pm)) / (2 * h)
         print(f"param {k}: fd {fdp:.6g} adjoint {gp[k]:.6g}", flush=True)
-        assert abs(fdp - gp[k]) <= 5e-2 * abs(gp[k])
+        assert abs(fdp - gp[k]) <= 1e-2 * abs(gp[k])
 
 
 def test_batch_equals_single_images(T, R):
