@@ -9,6 +9,7 @@ There is no CPU fallback: every compute call runs the HIP kernels, and importing
 raises if the shared library has not been built.
 """
 import ctypes as C
+import importlib.util
 import os
 
 import numpy as np
@@ -81,10 +82,26 @@ EXPORTED = [
 ]
 
 
+def _bind_torch_runtime():
+    """One process must hold one HIP runtime.  PyTorch ships its own libamdhip64 /
+    libhsa-runtime64 (same SONAMEs as /opt/rocm's); when PyTorch is installed, load those first
+    so that libbimp_hip.so binds to them by SONAME and a later `import torch` finds them
+    already loaded.  Device pointers and stream handles then pass freely between the two."""
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin or os.environ.get("BIMP_SYSTEM_HIP_RUNTIME"):
+        return
+    libdir = os.path.join(os.path.dirname(spec.origin), "lib")
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        path = os.path.join(libdir, name)
+        if os.path.exists(path):
+            C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `make -C cuda-bimp_amd` "
                           "or __graft_entry__.build()")
+    _bind_torch_runtime()
     L = C.CDLL(LIB_PATH)
     P, I, SZ = C.c_void_p, C.c_int, C.c_size_t
     sig = {
