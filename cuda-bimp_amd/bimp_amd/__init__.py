@@ -78,7 +78,7 @@ EXPORTED = [
     "bimp_batch_results", "bimp_batch_outputs", "bimp_batch_time", "bimp_batch_plan",
     "bimp_batch_last_error",
     "bimp_diff_create", "bimp_diff_destroy", "bimp_diff_response_size", "bimp_diff_set_params",
-    "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error",
+    "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error", "bimp_bgr_to_gray",
 ]
 
 
@@ -153,6 +153,7 @@ def _load():
         "bimp_diff_forward": (I, [P, P, I, P, P]),
         "bimp_diff_backward": (I, [P, P, I, P, P, P, P]),
         "bimp_diff_last_error": (C.c_char_p, [P]),
+        "bimp_bgr_to_gray": (I, [P, SZ, I, I, I, P, SZ, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -346,6 +347,19 @@ class CudaStuff:
         self._chk(lib.bimp_time_detect(self.ctx, d_img.ptr, pitch, d_out.ptr, cap, iters,
                                        stream.handle if stream else None, C.byref(ms)))
         return ms.value
+
+
+def bgr_to_gray(bgr):
+    """8-bit BGR (H, W, 3) -> gray (H, W) on the device (getGPUImage's cvtColor step)."""
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    h, w, c = bgr.shape
+    assert c == 3
+    d = DeviceBuffer(w * h)
+    _check(lib.bimp_bgr_to_gray(_ptr(bgr), 3 * w, w, h, 0, d.ptr, w, None))
+    out = np.zeros((h, w), np.uint8)
+    d.download(out)
+    d.free()
+    return out
 
 
 class Batch:

@@ -866,6 +866,30 @@ int bimp_time_detect(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypo
     return BIMP_OK;
 }
 
+int bimp_bgr_to_gray(const uint8_t* bgr, size_t bgr_pitch, int width, int height, int bgr_on_device,
+                     uint8_t* d_gray, size_t gray_pitch, bimp_stream_t stream) {
+    if (!bgr || !d_gray || width <= 0 || height <= 0 || bgr_pitch < (size_t)3 * width ||
+        gray_pitch < (size_t)width)
+        return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const uint8_t* src = bgr;
+    uint8_t* staged = nullptr;
+    if (!bgr_on_device) {
+        if (hipMalloc(&staged, bgr_pitch * height) != hipSuccess) return BIMP_E_ALLOC;
+        if (hipMemcpyAsync(staged, bgr, bgr_pitch * height, hipMemcpyHostToDevice, s) != hipSuccess) {
+            (void)hipFree(staged);
+            return BIMP_E_DEVICE;
+        }
+        src = staged;
+    }
+    hipError_t e = bimpk::launch_bgr_to_gray(src, bgr_pitch, width, height, d_gray, gray_pitch, s);
+    if (staged) {
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(staged);
+    }
+    return e == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+
 int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t stream) {
     if (!dst || !src) return BIMP_E_ARG;
     if (bimpk::launch_copy(dst, src, bytes, (hipStream_t)stream) != hipSuccess) return BIMP_E_DEVICE;

@@ -18,8 +18,7 @@
 // A K-step is keyed by an aligned block of 4 image rows: the image fragment of rows
 // 4*rho .. 4*rho+3 serves every output row j of the wavefront, with the tap block
 // u0 = 4*rho - j (filter fragments precomputed for every u0).  So each fragment read from
-// LDS feeds kDirRows x kDirGroups MFMAs and each filter fragment kDirGroups of them; the
-// filter fragments of a row block are staged in LDS once for all wavefronts.
+// LDS feeds kDirRows x kDirGroups MFMAs and each filter fragment kDirGroups of them.
 // The image tile sits in LDS as four copies shifted by 0..3 samples: every lane's 8-sample
 // window is then two aligned 8-byte reads.
 #include <hip/hip_runtime.h>
@@ -74,7 +73,7 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     constexpr int M = kDirRows, NCG = kDirGroups, TY = kDirTY, TX = kDirTX;
     extern __shared__ unsigned char smem[];
     T* copies = reinterpret_cast<T*>(smem);
-    const int LH = ds.LH, LWP = ds.lwp, csize = LH * LWP;
+    const int LWP = ds.lwp, csize = ds.cstride;
     float* vs = reinterpret_cast<float*>(smem + (size_t)4 * csize * sizeof(T));
     const int k = ds.kpad, kW = ds.kW;
     const int TWi = TX + 2 * k;  // columns of the image window the tile needs
@@ -85,7 +84,7 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     // position c - s of copy s).  Positions past the image window hold zero; they only meet
     // zero taps.
     const bool inside = y0 - k >= 0 && y0 + TY + k <= ds.H && x0 - k >= 0 && x0 + TX + k <= ds.W;
-    for (int r = wave; r < LH; r += kDirWaves) {
+    for (int r = wave; r < ds.LH; r += kDirWaves) {
         const bool rowin = r < TY + 2 * k;
         const int gy = !rowin ? 0 : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
         const uint8_t* row = img + (size_t)gy * pitch;
@@ -103,77 +102,9 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     }
     __syncthreads();
 
-    // Main loop.  Lane (p, kb): pixel column p of each 16-column group, tap row kb of the
-    // K-step.  The copy index s = (off + p) & 3 is fixed per lane.
-    const int p = lane & 15, kb = lane >> 4;
-    const int s = (ds.off + p) & 3;
-    const T* lanebase = copies + s * csize + (ds.off + p - s) + (wave * M + ds.off + kb) * LWP;
-    f32x4 acc[M][NCG];
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-#pragma unroll
-        for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // The filter fragments of one 4-row block rho (tap blocks u0 = 4*rho - j, j < M: rows
-    // 4*rho .. 4*rho+3 of the fragment table, contiguous) are shared by all wavefronts: staged
-    // in LDS per block, the next block's fetched into registers while this one is used.
-    const int nvb = ds.nvb;
-    v8* bl = reinterpret_cast<v8*>(vs);
-    const int chunk = 4 * nvb * 64;  // fragments per block
-    constexpr int kPre = (4 * 7 * 64 + kDirWaves * 64 - 1) / (kDirWaves * 64);
-    v8 pre[kPre];
-#pragma unroll
-    for (int i = 0; i < kPre; ++i) {
-        const int e = tid + i * kDirWaves * 64;
-        if (e < chunk) pre[i] = bfrag[e];
-    }
-    v8 a[NCG];
-    auto load_a = [&](int rho, int vb) {
-        const T* rowp = lanebase + 4 * rho * LWP + 8 * vb;
-#pragma unroll
-        for (int g = 0; g < NCG; ++g) {
-            const v4 lo = *reinterpret_cast<const v4*>(rowp + 16 * g);
-            const v4 hi = *reinterpret_cast<const v4*>(rowp + 16 * g + 4);
-            a[g] = v8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-        }
-    };
-#pragma unroll 1
-    for (int rho = 0; rho < ds.nrho; ++rho) {
-        __syncthreads();  // every wavefront is done with the previous block's fragments
-#pragma unroll
-        for (int i = 0; i < kPre; ++i) {
-            const int e = tid + i * kDirWaves * 64;
-            if (e < chunk) bl[e] = pre[i];
-        }
-        __syncthreads();
-        if (rho + 1 < ds.nrho) {
-            const v8* src = bfrag + (size_t)(rho + 1) * chunk;
-#pragma unroll
-            for (int i = 0; i < kPre; ++i) {
-                const int e = tid + i * kDirWaves * 64;
-                if (e < chunk) pre[i] = src[e];
-            }
-        }
-        load_a(rho, 0);
-#pragma unroll 1
-        for (int vb = 0; vb < nvb; ++vb) {
-            v8 b[M];
-#pragma unroll
-            for (int j = 0; j < M; ++j) b[j] = bl[((M - 1 - j) * nvb + vb) * 64 + lane];
-            v8 cur[NCG];
-#pragma unroll
-            for (int g = 0; g < NCG; ++g) cur[g] = a[g];
-            if (vb + 1 < nvb) load_a(rho, vb + 1);
-#pragma unroll
-            for (int j = 0; j < M; ++j)
-#pragma unroll
-                for (int g = 0; g < NCG; ++g) acc[j][g] = Mfma<T>::run(cur[g], b[j], acc[j][g]);
-        }
-    }
-    __syncthreads();  // the fragment area is reused for the box sums below
-
-    // kW x kW box sums of the tile's output pixels, in the fragment area: vertical running
-    // sums over kW rows into vs[r][c] (two segments of TY/2 rows per column), then horizontal
-    // ones, stored back into vs as a TY x TX array.
+    // kW x kW box sums of the tile's output pixels: vertical running sums over kW rows into
+    // vs[r][c] (two segments of TY/2 rows per column), then horizontal ones, stored back into
+    // vs as a TY x TX array.
     if (tid < 2 * TWi) {
         const int seg = tid >= TWi ? 1 : 0;
         const int c = tid - seg * TWi, r0 = seg * (TY / 2);
@@ -208,7 +139,55 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
         dstb[1] = make_float4(bsum[4], bsum[5], bsum[6], bsum[7]);
     }
 
-    __syncthreads();
+    // Main loop.  Lane (p, kb): pixel column p of each 16-column group, tap row kb of the
+    // K-step.  The copy index s = (off + p) & 3 is fixed per lane.
+    const int p = lane & 15, kb = lane >> 4;
+    const int s = (ds.off + p) & 3;
+    const T* lanebase = copies + s * csize + (ds.off + p - s) + (wave * M + ds.off + kb) * LWP;
+    f32x4 acc[M][NCG];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Software-pipelined over the K-steps it = rho * nvb + vb: the image and filter
+    // fragments of step it + 1 are in flight while the MFMAs of step it run.
+    const int nvb = ds.nvb, nsteps = ds.nrho * nvb;
+    auto load_a = [&](int it, v8* a) {
+        const int rho = it / nvb, vb = it - rho * nvb;
+        const T* rowp = lanebase + 4 * rho * LWP + 8 * vb;
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) {
+            const v4 lo = *reinterpret_cast<const v4*>(rowp + 16 * g);
+            const v4 hi = *reinterpret_cast<const v4*>(rowp + 16 * g + 4);
+            a[g] = v8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+    };
+    auto load_b = [&](int it, v8* b) {
+        const int rho = it / nvb, vb = it - rho * nvb;
+        // filter fragments of u0 = 4*rho - j sit at row index u0 + M - 1
+        const v8* bro = bfrag + ((size_t)(4 * rho + M - 1) * nvb + vb) * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < M; ++j) b[j] = bro[-j * nvb * 64];
+    };
+    v8 a[NCG], b[M];
+    load_a(0, a);
+    load_b(0, b);
+#pragma unroll 1
+    for (int it = 0; it < nsteps; ++it) {
+        v8 an[NCG], bn[M];
+        const int nx = it + 1 < nsteps ? it + 1 : it;
+        load_a(nx, an);
+        load_b(nx, bn);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int g = 0; g < NCG; ++g) acc[j][g] = Mfma<T>::run(a[g], b[j], acc[j][g]);
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) a[g] = an[g];
+#pragma unroll
+        for (int j = 0; j < M; ++j) b[j] = bn[j];
+    }
+    __syncthreads();  // the box sums stored before the main loop are read below
 
     // Epilogue: lane holds filter jf = lane%16 for pixels 4*kb .. 4*kb+3 of each group; the
     // odd filter of orientation jf/2 sits in the neighbouring lane.
@@ -259,9 +238,11 @@ size_t direct_geometry(DirectScale& d) {
     const int tws = std::max(twi, d.off + kDirTX + extp - 1);
     d.lwp = (tws + 4 + 3) / 4 * 4;
     d.vsp = twi;
-    const size_t frag = (size_t)4 * d.nvb * 64 * 16;  // one block's filter fragments
-    const size_t box = (size_t)kDirTY * d.vsp * sizeof(float);
-    return (size_t)4 * d.LH * d.lwp * 2 + std::max(frag, box);
+    // copy stride: 16 (mod 64) elements puts the four shifted copies 8 LDS banks apart, so the
+    // 16 lanes of one 8-byte read (4 copies x 4 windows) hit 32 distinct banks
+    const int cs = d.LH * d.lwp;
+    d.cstride = cs + ((16 - cs % 64) + 64) % 64;
+    return (size_t)4 * d.cstride * 2 + (size_t)kDirTY * d.vsp * sizeof(float);
 }
 
 hipError_t prepare_direct() {
@@ -278,7 +259,7 @@ hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectScale& ds
     const size_t lds = direct_geometry(d);
     // the geometry must be the one the filter fragments were built for
     if (lds > kMaxLds || d.off != ds.off || d.nvb != ds.nvb || d.nrho != ds.nrho ||
-        d.LH != ds.LH || d.lwp != ds.lwp || d.vsp != ds.vsp || ds.nvb > 7 ||
+        d.LH != ds.LH || d.lwp != ds.lwp || d.vsp != ds.vsp || d.cstride != ds.cstride ||
         2 * (kDirTX + 2 * ds.kpad) > kDirWaves * 64 || ds.kW != 2 * ds.kpad + 1 || ds.off < 0)
         return hipErrorInvalidValue;
     dim3 grid((ds.Wc + kDirTX - 1) / kDirTX, (ds.Hc + kDirTY - 1) / kDirTY);

@@ -740,6 +740,18 @@ __global__ void k_init(unsigned* maxslots, int nslots, int* counter) {
     if (i == 0) *counter = 0;
 }
 
+// 8-bit BGR -> gray with the fixed-point weights of OpenCV's COLOR_BGR2GRAY (0.114, 0.587,
+// 0.299 in Q14: 1868, 9617, 4899; rounding by 2^13), as getGPUImage uses it
+// (cuda_bimp.cu:170-174).
+__global__ void k_bgr_to_gray(const uint8_t* __restrict__ bgr, size_t bpitch, int w, int h,
+                              uint8_t* __restrict__ gray, size_t gpitch) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const uint8_t* p = bgr + (size_t)y * bpitch + 3 * (size_t)x;
+    const unsigned v = 1868u * p[0] + 9617u * p[1] + 4899u * p[2] + (1u << 13);
+    gray[(size_t)y * gpitch + x] = (uint8_t)(v >> 14);
+}
+
 __global__ void k_store_count(const int* __restrict__ counter, int* __restrict__ dst) {
     if (threadIdx.x == 0) dst[0] = counter[0];
 }
@@ -1006,6 +1018,12 @@ hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_block
                             void* out, int cap, int matrix_ld, hipStream_t s) {
     k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, maxslots, thresh,
                                                      block_offsets, out, cap, matrix_ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_bgr_to_gray(const uint8_t* bgr, size_t bpitch, int w, int h, uint8_t* gray,
+                              size_t gpitch, hipStream_t s) {
+    k_bgr_to_gray<<<dim3((w + 255) / 256, h), 256, 0, s>>>(bgr, bpitch, w, h, gray, gpitch);
     return hipGetLastError();
 }
 

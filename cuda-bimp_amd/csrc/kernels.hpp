@@ -51,6 +51,7 @@ struct DirectScale {
     int Hc, Wc, ldc;         // complex-cell map region and pitch
     int off, ext, nvb, nrho; // tap window offset kX - R, extent, 8-column blocks, 4-row blocks
     int LH, lwp, vsp;        // staged rows, LDS row pitch (elements), box-sum row pitch (floats)
+    int cstride;             // element stride between the four shifted copies
     float fscale, iscale;    // filter scale, input scale (255/65536)
     float dc[2 * kNumOri];   // box coefficients, filter 2o even, 2o+1 odd
 };
@@ -98,6 +99,8 @@ hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_block
                             const unsigned* maxslots, double thresh, const int* block_offsets,
                             void* out, int cap, int matrix_ld, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+hipError_t launch_bgr_to_gray(const uint8_t* bgr, size_t bpitch, int w, int h, uint8_t* gray,
+                              size_t gpitch, hipStream_t s);
 // dst[0] = counter[0] on the stream (a kernel, so that it can sit in a captured graph).
 hipError_t launch_store_count(const int* counter, int* dst, hipStream_t s);
 

@@ -212,6 +212,12 @@ int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t str
 int bimp_synth_frame(int width, int height, int nshapes, uint64_t seed, uint8_t* out,
                      size_t pitch);
 
+/* getGPUImage's colour step (cuda_bimp.cu:156-179, cv::cuda::cvtColor COLOR_BGR2GRAY): 8-bit
+ * interleaved BGR (host or device) -> 8-bit gray on the device, with OpenCV's Q14 weights.
+ * The MIN_SIZE upscale of getGPUImage is not provided. */
+int bimp_bgr_to_gray(const uint8_t* bgr, size_t bgr_pitch, int width, int height, int bgr_on_device,
+                     uint8_t* d_gray, size_t gray_pitch, bimp_stream_t stream);
+
 /* ---- Batching form --------------------------------------------------------------------
  * Many frames of uneven sizes in one launch.  The reference has no batch entry: its caller
  * runs getKeypoints frame by frame (example_bimp.cpp:48-54, cuda_bimp.cu:109-140); a batch

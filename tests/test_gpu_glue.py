@@ -1,0 +1,34 @@
+"""GPU test of getGPUImage's colour step (bimp_bgr_to_gray, cuda_bimp.cu:156-179).
+
+OpenCV is not installed, so parity with cv::cuda::cvtColor is pinned by what its Q14 weights
+fix: grey pixels are unchanged (the weights sum to 2^14), and each pure channel maps to
+floor(255 * w / 2^14 + 1/2) with w = 1868 (B), 9617 (G), 4899 (R)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def B():
+    import bimp_amd
+    if bimp_amd.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP library")
+    return bimp_amd
+
+
+def test_grey_pixels_unchanged(B):
+    v = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    bgr = np.repeat(v[:, :, None], 3, axis=2)
+    assert np.array_equal(B.bgr_to_gray(bgr), v)
+
+
+def test_channel_order_and_rounding(B):
+    bgr = np.zeros((1, 4, 3), np.uint8)
+    bgr[0, 0] = (255, 0, 0)      # blue
+    bgr[0, 1] = (0, 255, 0)      # green
+    bgr[0, 2] = (0, 0, 255)      # red
+    bgr[0, 3] = (10, 20, 30)
+    g = B.bgr_to_gray(bgr)[0]
+    assert list(g[:3]) == [29, 150, 76]
+    assert g[3] == (1868 * 10 + 9617 * 20 + 4899 * 30 + 8192) >> 14
