@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(kRowThreads) k_c2c_rows(float2* __restrict__ d
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = p[in_index<N>(e, lane)];
-    bimpdev::fft<N, INV>(v, lds + grp * N, lane, tw);
+    bimpdev::fft<N, INV>(v, lds + grp * bimpdev::fft_lds_floats<N>(), lane, tw);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) p[out_index<N>(e, lane)] = v[e];
 }
@@ -380,7 +380,7 @@ inline dim3 ew(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 hipError_t diff_prepare(int n) {
 #define X(N)                                                                                 \
     if (n == N) {                                                                            \
-        const int lds = (kRowThreads / FFTCfg<N>::TPV) * N * (int)sizeof(float);             \
+        const int lds = (kRowThreads / FFTCfg<N>::TPV) * bimpdev::fft_lds_floats<N>() * (int)sizeof(float); \
         hipError_t e = hipFuncSetAttribute((const void*)k_c2c_rows<N, false>,                \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
         if (e != hipSuccess) return e;                                                       \
@@ -396,7 +396,7 @@ hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, 
 #define X(N)                                                                                 \
     if (n == N) {                                                                            \
         constexpr int GPB = kRowThreads / FFTCfg<N>::TPV;                                    \
-        const int lds = GPB * N * (int)sizeof(float);                                        \
+        const int lds = GPB * bimpdev::fft_lds_floats<N>() * (int)sizeof(float);             \
         dim3 grid((rows + GPB - 1) / GPB);                                                   \
         if (inv)                                                                             \
             k_c2c_rows<N, true><<<grid, kRowThreads, lds, s>>>(data, rows, tw);              \

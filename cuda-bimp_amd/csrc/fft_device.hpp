@@ -255,9 +255,16 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
     }
 }
 
-// The exchange moves real parts, then imaginary parts, through an LDS region of N floats:
-// a register slot keeps its old imaginary part until the second half has been written, so
-// half the LDS of a complex exchange suffices.
+// LDS floats one group's exchange region needs, and the padded address of element i: one
+// pad word per 32 keeps the stride-R writes of the first stages (lane j writes j*R + r) from
+// landing on a handful of banks (stride 16 alone would put 32 lanes on 2 banks).
+template <int N>
+__host__ __device__ constexpr int fft_lds_floats() { return N + N / 32; }
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+
+// The exchange moves real parts, then imaginary parts, through an LDS region of
+// fft_lds_floats<N>() floats: a register slot keeps its old imaginary part until the second
+// half has been written, so half the LDS of a complex exchange suffices.
 template <int N, int Ns, int R, int R2>
 __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, int lane) {
     using C = FFTCfg<N>;
@@ -266,7 +273,7 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
         const int j = lane + C::TPV * m;
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[base + r * Ns] = v[m * R + r].x;
+        for (int r = 0; r < R; ++r) lds[lds_pad(base + r * Ns)] = v[m * R + r].x;
     }
     stage_sync();
     float im[C::E];
@@ -276,7 +283,7 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
     for (int m = 0; m < C::E / R2; ++m) {
         const int j = lane + C::TPV * m;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[j + r * (N / R2)];
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[lds_pad(j + r * (N / R2))];
     }
     stage_sync();
 #pragma unroll
@@ -284,14 +291,14 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
         const int j = lane + C::TPV * m;
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[base + r * Ns] = im[m * R + r];
+        for (int r = 0; r < R; ++r) lds[lds_pad(base + r * Ns)] = im[m * R + r];
     }
     stage_sync();
 #pragma unroll
     for (int m = 0; m < C::E / R2; ++m) {
         const int j = lane + C::TPV * m;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[j + r * (N / R2)];
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[lds_pad(j + r * (N / R2))];
     }
     stage_sync();
 }

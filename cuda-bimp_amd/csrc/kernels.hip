@@ -171,7 +171,7 @@ struct InvColCfg {
     static constexpr int BR = FH / RL;
     static constexpr int SS = BR + 8;  // staging stride (float2); == 8 (mod 32) when 32 | BR
     static constexpr size_t STAGE = (size_t)CB * SS * sizeof(float2);
-    static constexpr size_t EXCH = (size_t)CB * FH * sizeof(float);
+    static constexpr size_t EXCH = (size_t)CB * fft_lds_floats<FH>() * sizeof(float);
     static constexpr size_t LDS = STAGE > EXCH ? STAGE : EXCH;
 };
 
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
     const int half = FW / 2;
     const int og = o <= 4 ? o : kNumOri - o;
     const int g = threadIdx.x / C::TPV, lane0 = threadIdx.x % C::TPV;
-    float* my = reinterpret_cast<float*>(lds2) + (size_t)g * FH;
+    float* my = reinterpret_cast<float*>(lds2) + (size_t)g * fft_lds_floats<FH>();
 
 #pragma unroll 1
     for (int partner = 0; partner < 2; ++partner) {
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __re
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = row[in_index<FW>(e, lane)];
-    fft<FW, true>(v, lds + (size_t)g * FW, lane, tw);
+    fft<FW, true>(v, lds + (size_t)g * fft_lds_floats<FW>(), lane, tw);
     if (active) {
         float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
 #pragma unroll
@@ -929,7 +929,7 @@ hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t t
     if (FW == N) {                                                                           \
         const int vpb = RowCfg<N>::VPB;                                                      \
         dim3 grid((Hc + vpb - 1) / vpb, kNumOri, ss.n);                                      \
-        k_inv_rows<N><<<grid, RowCfg<N>::THREADS, vpb * N * sizeof(float), s>>>(             \
+        k_inv_rows<N><<<grid, RowCfg<N>::THREADS, vpb * fft_lds_floats<N>() * sizeof(float), s>>>( \
             ss, T, tstride, Hc, Wc, ldc, tw);                                                \
         return hipGetLastError();                                                            \
     }
