@@ -101,6 +101,7 @@ void free_all(bimp_ctx* c) {
         (void)hipFree(s.frag);
     }
     for (auto* p : c->level) (void)hipFree(p);
+    for (auto& st : c->steps) (void)hipFree(st.jobs);
     for (auto& kv : c->tw) (void)hipFree(kv.second);
     (void)hipFree(c->R);
     (void)hipFree(c->T);
@@ -127,6 +128,9 @@ int launch_cells_scale(bimp_ctx* c, size_t i, hipStream_t s) {
 // Budget for the inverse-pass intermediate T: scales of one group share an inverse launch
 // while their T stays within it, so that T is consumed from the 256 MB last-level cache.
 constexpr size_t kTBudgetBytes = 160u << 20;
+
+// Direct-form scales of one level per launch (fills the chip's tail with the lighter scales).
+constexpr int kDirectLaunchScales = 4;
 
 int inverse_batch(size_t tstride_elems) {
     const size_t per = tstride_elems * sizeof(float2);
@@ -169,19 +173,25 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
     }
     for (const Step& st : c->steps) {
         if (st.direct) {
-            const int i = st.index;
-            const Scale& sk = c->sc[i];
-            const auto& g = sk.g;
+            const Scale& s0 = c->sc[st.index];
+            const auto& g = s0.g;
             {
-                LaunchScope ls(c, s, F_DIRECT,
-                               (double)g.H * g.W + 4.0 * kNumOri * sk.Hc * sk.Wc, sk.ds.ext);
-                HIPCHK(c, bimpk::launch_direct(lv[g.level], lp[g.level], sk.ds,
-                                               sk.precision == BIMP_PRECISION_FP16, sk.frag,
-                                               sk.C, s));
+                double bytes = (double)g.H * g.W;
+                int maxext = 0;
+                for (int k = 0; k < st.n; ++k) {
+                    const Scale& sk = c->sc[st.index + k];
+                    bytes += 4.0 * kNumOri * sk.Hc * sk.Wc;
+                    maxext = std::max(maxext, sk.ds.ext);
+                }
+                LaunchScope ls(c, s, F_DIRECT, bytes, maxext);
+                HIPCHK(c, bimpk::launch_direct(lv[g.level], lp[g.level], st.jobs, st.n, st.tiles,
+                                               st.lds, s0.precision == BIMP_PRECISION_FP16, s));
             }
             if (cells) {
-                const int r = launch_cells_scale(c, (size_t)i, s);
-                if (r) return r;
+                for (int k = 0; k < st.n; ++k) {
+                    const int r = launch_cells_scale(c, (size_t)(st.index + k), s);
+                    if (r) return r;
+                }
             }
             continue;
         }
@@ -470,7 +480,18 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     for (int i = 0; i < (int)c->sc.size(); ++i) {
         const auto& g = c->sc[i].g;
         if (c->sc[i].precision != BIMP_PRECISION_FP32) {
-            c->steps.push_back({true, i});
+            // consecutive direct scales of one level share a launch
+            if (!c->steps.empty() && c->steps.back().direct && !prevFft) {
+                Step& last = c->steps.back();
+                if (c->sc[last.index].g.level == g.level && last.n < kDirectLaunchScales) {
+                    ++last.n;
+                    continue;
+                }
+            }
+            Step st;
+            st.direct = true;
+            st.index = i;
+            c->steps.push_back(st);
             prevFft = false;
             continue;
         }
@@ -553,6 +574,25 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     if (anyDirect) {
         if (hipError_t e = bimpk::prepare_direct(); e != hipSuccess)
             return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));
+        for (Step& st : c->steps) {
+            if (!st.direct) continue;
+            std::vector<bimpk::DirectScale> ds;
+            std::vector<const void*> fr;
+            std::vector<float*> cm;
+            for (int k = 0; k < st.n; ++k) {
+                const Scale& sk = c->sc[st.index + k];
+                ds.push_back(sk.ds);
+                fr.push_back(sk.frag);
+                cm.push_back(sk.C);
+            }
+            std::vector<bimpk::DirectJob> jobs(st.n);
+            st.lds = bimpk::direct_jobs(ds.data(), fr.data(), cm.data(), st.n, jobs.data(), &st.tiles);
+            if (!st.lds) return bail(BIMP_E_UNSUPPORTED, "direct-form job table");
+            if (dalloc(&st.jobs, jobs.size()) != hipSuccess) return bail(BIMP_E_ALLOC, "direct jobs");
+            if (hipMemcpy(st.jobs, jobs.data(), jobs.size() * sizeof(bimpk::DirectJob),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return bail(BIMP_E_DEVICE, "direct jobs upload");
+        }
     }
     if (hipError_t e = bimpk::prepare_common(); e != hipSuccess)
         return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));

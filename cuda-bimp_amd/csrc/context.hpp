@@ -42,7 +42,11 @@ struct Scale {
 // One step of a frame: an FFT resolution group or one direct-form scale.
 struct Step {
     bool direct;
-    int index;  // group index or scale index
+    int index;  // FFT: group index; direct: first scale
+    int n = 1;  // direct: consecutive scales of one level in one launch
+    bimpk::DirectJob* jobs = nullptr;  // direct: device job table
+    int tiles = 0;
+    size_t lds = 0;
 };
 
 // Scales sharing pyramid level and transform size; each FFT stage runs once per group.

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -64,20 +65,30 @@ __device__ __forceinline__ int pad_index(int i, int n, int F, int k) {
     return i < n ? i : (i < n + k ? n - 1 : 0);
 }
 
+// One launch covers every scale of a job table (scales of one pyramid level, heaviest first):
+// workgroup b works on the scale whose tile range [tile0, tile0 + tiles) holds b.
 template <class T>
 __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
-        const uint8_t* __restrict__ img, size_t pitch, DirectScale ds,
-        const typename Mfma<T>::v8* __restrict__ bfrag, float* __restrict__ Cm) {
+        const uint8_t* __restrict__ img, size_t pitch, const DirectJob* __restrict__ jobs,
+        int njobs) {
     typedef typename Mfma<T>::v8 v8;
     typedef typename Mfma<T>::v4 v4;
     constexpr int M = kDirRows, NCG = kDirGroups, TY = kDirTY, TX = kDirTX;
+    int js = njobs - 1;
+    while (js > 0 && (int)blockIdx.x < jobs[js].tile0) --js;  // uniform
+    const DirectJob& job = jobs[js];
+    const DirectScale& ds = job.ds;
+    const v8* __restrict__ bfrag = reinterpret_cast<const v8*>(job.frag);
+    float* __restrict__ Cm = job.C;
+    const int tb = (int)blockIdx.x - job.tile0;
+    const int tby = tb / job.tiles_x, tbx = tb - tby * job.tiles_x;
     extern __shared__ unsigned char smem[];
     T* copies = reinterpret_cast<T*>(smem);
     const int LWP = ds.lwp, csize = ds.cstride;
     float* vs = reinterpret_cast<float*>(smem + (size_t)4 * csize * sizeof(T));
     const int k = ds.kpad, kW = ds.kW;
     const int TWi = TX + 2 * k;  // columns of the image window the tile needs
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = tbx * TX, y0 = tby * TY;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
     // Stage rows y0-k .. and columns x0-k .. of Pd as four shifted copies (tile sample c at
@@ -253,20 +264,44 @@ hipError_t prepare_direct() {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
 }
 
-hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectScale& ds, bool fp16,
-                         const void* bfrag, float* C, hipStream_t s) {
-    DirectScale d = ds;
-    const size_t lds = direct_geometry(d);
-    // the geometry must be the one the filter fragments were built for
-    if (lds > kMaxLds || d.off != ds.off || d.nvb != ds.nvb || d.nrho != ds.nrho ||
-        d.LH != ds.LH || d.lwp != ds.lwp || d.vsp != ds.vsp || d.cstride != ds.cstride ||
-        2 * (kDirTX + 2 * ds.kpad) > kDirWaves * 64 || ds.kW != 2 * ds.kpad + 1 || ds.off < 0)
-        return hipErrorInvalidValue;
-    dim3 grid((ds.Wc + kDirTX - 1) / kDirTX, (ds.Hc + kDirTY - 1) / kDirTY);
+size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const* C, int n,
+                   DirectJob* jobs, int* total_tiles) {
+    // heaviest scales first, so that the short ones fill the tail of the launch
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (long)ds[a].ext * ds[a].ext > (long)ds[b].ext * ds[b].ext;
+    });
+    size_t lds = 0;
+    int t = 0;
+    for (int k = 0; k < n; ++k) {
+        const int i = order[k];
+        DirectScale d = ds[i];
+        const size_t l = direct_geometry(d);
+        if (l > kMaxLds || d.off != ds[i].off || d.nvb != ds[i].nvb || d.nrho != ds[i].nrho ||
+            d.LH != ds[i].LH || d.lwp != ds[i].lwp || d.vsp != ds[i].vsp ||
+            d.cstride != ds[i].cstride || 2 * (kDirTX + 2 * ds[i].kpad) > kDirWaves * 64 ||
+            ds[i].kW != 2 * ds[i].kpad + 1 || ds[i].off < 0)
+            return 0;
+        lds = std::max(lds, l);
+        jobs[k].ds = ds[i];
+        jobs[k].frag = frag[i];
+        jobs[k].C = C[i];
+        jobs[k].tiles_x = (ds[i].Wc + kDirTX - 1) / kDirTX;
+        jobs[k].tile0 = t;
+        t += jobs[k].tiles_x * ((ds[i].Hc + kDirTY - 1) / kDirTY);
+    }
+    *total_tiles = t;
+    return lds;
+}
+
+hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectJob* d_jobs, int njobs,
+                         int total_tiles, size_t lds, bool fp16, hipStream_t s) {
+    if (lds > kMaxLds || njobs < 1 || total_tiles < 1) return hipErrorInvalidValue;
     if (fp16)
-        k_direct<_Float16><<<grid, kDirWaves * 64, lds, s>>>(img, pitch, ds, (const f16x8*)bfrag, C);
+        k_direct<_Float16><<<total_tiles, kDirWaves * 64, lds, s>>>(img, pitch, d_jobs, njobs);
     else
-        k_direct<__bf16><<<grid, kDirWaves * 64, lds, s>>>(img, pitch, ds, (const bf16x8*)bfrag, C);
+        k_direct<__bf16><<<total_tiles, kDirWaves * 64, lds, s>>>(img, pitch, d_jobs, njobs);
     return hipGetLastError();
 }
 

@@ -255,12 +255,22 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
     }
 }
 
-// LDS floats one group's exchange region needs, and the padded address of element i: one
-// pad word per 32 keeps the stride-R writes of the first stages (lane j writes j*R + r) from
-// landing on a handful of banks (stride 16 alone would put 32 lanes on 2 banks).
+// LDS floats one group's exchange region needs.  After the first stage lane j writes its R
+// outputs at j*R + r: with R = 16 that stride would put 32 lanes on 2 banks, so that exchange
+// stores element i at i + i/R (an odd lane stride R + 1; the offsets of one lane and of the
+// reads' r-steps stay compile-time constants, so the accesses still pair up).  Later exchanges
+// write runs of >= R consecutive lanes and keep the plain layout.
 template <int N>
-__host__ __device__ constexpr int fft_lds_floats() { return N + N / 32; }
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int fft_lds_floats() {
+    return N + N / RadInfo<typename FFTCfg<N>::Rad>::first;
+}
+template <int Ns, int R>
+__device__ __forceinline__ int lds_pad(int i) {
+    if constexpr (Ns == 1)
+        return i + i / R;
+    else
+        return i;
+}
 
 // The exchange moves real parts, then imaginary parts, through an LDS region of
 // fft_lds_floats<N>() floats: a register slot keeps its old imaginary part until the second
@@ -273,7 +283,7 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
         const int j = lane + C::TPV * m;
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_pad(base + r * Ns)] = v[m * R + r].x;
+        for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R>(base + r * Ns)] = v[m * R + r].x;
     }
     stage_sync();
     float im[C::E];
@@ -283,7 +293,7 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
     for (int m = 0; m < C::E / R2; ++m) {
         const int j = lane + C::TPV * m;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[lds_pad(j + r * (N / R2))];
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[lds_pad<Ns, R>(j + r * (N / R2))];
     }
     stage_sync();
 #pragma unroll
@@ -291,14 +301,14 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
         const int j = lane + C::TPV * m;
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_pad(base + r * Ns)] = im[m * R + r];
+        for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R>(base + r * Ns)] = im[m * R + r];
     }
     stage_sync();
 #pragma unroll
     for (int m = 0; m < C::E / R2; ++m) {
         const int j = lane + C::TPV * m;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[lds_pad(j + r * (N / R2))];
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[lds_pad<Ns, R>(j + r * (N / R2))];
     }
     stage_sync();
 }

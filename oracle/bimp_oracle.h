@@ -84,6 +84,10 @@ int bo_cells_scale(const bo_ctx* c, int i, const float* maps, float* D, float* S
 /* Stage 3: NMS of D for scale i; appends at out[*count..] (writes only below cap), *count
  * grows by the number of keypoints found. */
 int bo_nms_scale(const bo_ctx* c, int i, const float* D, bo_keypoint* out, int cap, int* count);
+/* The reference's CPU keypoint mode for scale i (cuda_bimp.cu:745-808) on the D and S maps;
+ * appends the D keypoints (class_id -1) like bo_nms_scale. */
+int bo_nms_cpu_mode(const bo_ctx* c, int i, const float* D, const float* S, bo_keypoint* out,
+                    int cap, int* count);
 /* Whole frame: stages 0-3 over every scale in ascending lambda order.  D_out / S_out, if
  * non-null, receive the per-scale maps concatenated (sum of dataH*dataW). */
 int bo_detect(bo_ctx* c, const uint8_t* img, size_t pitch, bo_keypoint* out, int cap,
