@@ -79,6 +79,7 @@ EXPORTED = [
     "bimp_batch_last_error",
     "bimp_diff_create", "bimp_diff_destroy", "bimp_diff_response_size", "bimp_diff_set_params",
     "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error", "bimp_bgr_to_gray",
+    "bimp_keypoints_cpu_mode",
 ]
 
 
@@ -154,6 +155,7 @@ def _load():
         "bimp_diff_backward": (I, [P, P, I, P, P, P, P]),
         "bimp_diff_last_error": (C.c_char_p, [P]),
         "bimp_bgr_to_gray": (I, [P, SZ, I, I, I, P, SZ, P]),
+        "bimp_keypoints_cpu_mode": (I, [P, P, I, C.POINTER(I), P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -329,6 +331,13 @@ class CudaStuff:
                                    arr.shape[1], 0, stream))
 
     # ---- profiling
+    def keypoints_cpu_mode(self, cap=KP_MAX_POINTS, stream=None):
+        """The reference's CPU keypoint mode on the current D and S maps."""
+        out = np.zeros(max(cap, 1), KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        self._chk(lib.bimp_keypoints_cpu_mode(self.ctx, _ptr(out), cap, C.byref(n), stream))
+        return out[:min(n.value, cap)].copy(), n.value
+
     def set_flags(self, flags):
         self._chk(lib.bimp_set_flags(self.ctx, flags))
 

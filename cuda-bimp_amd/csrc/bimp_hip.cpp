@@ -906,6 +906,84 @@ int bimp_time_detect(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypo
     return BIMP_OK;
 }
 
+// CPU keypoint mode, cuda_keypoints (cuda_bimp.cu:745-808): the reference's getKeypoints with
+// cpu_mode (cuda_bimp.cu:117-124) downloads D and S and extracts keypoints on the host with
+// its own rules: 8-neighbour dilation ignoring outside neighbours, threshold 0.104*max(D) for
+// both maps, rows and columns in [off, n - off), and a keypoint position that adds the S-map
+// parabola offsets to the D-map ones when both maps peak at a pixel.
+namespace {
+float parabola_peak_cpu(float v1, float v2, float v3) {
+    // parabolaPeak with positions (-1, 0, 1) (cuda_bimp.cu:689-711)
+    const float b = (v3 - v1) * 0.5f;
+    const float a = v1 + b - v2;
+    return -b / (a * 2);
+}
+float neighbour_max(const std::vector<float>& m, int W, int H, int x, int y) {
+    float r = -INFINITY;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = x + dx;
+            if ((!dx && !dy) || xx < 0 || xx >= W) continue;
+            r = std::max(r, m[(size_t)yy * W + xx]);
+        }
+    }
+    return r;
+}
+}  // namespace
+
+int bimp_keypoints_cpu_mode(bimp_ctx* c, bimp_keypoint* out, int cap, int* count,
+                            bimp_stream_t stream) {
+    if (!c || !count || (cap > 0 && !out)) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    int total = 0;
+    for (const Scale& sk : c->sc) {
+        const auto& g = sk.g;
+        const int H = g.H, W = g.W;
+        std::vector<float> Dm((size_t)H * W), Sm((size_t)H * W);
+        HIPCHK(c, hipMemcpyAsync(Dm.data(), sk.D, Dm.size() * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(Sm.data(), sk.S, Sm.size() * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        float maxd = Dm[0];
+        for (float v : Dm) maxd = std::max(maxd, v);
+        const double threshd = 0.104 * (double)maxd;
+        const int off = (int)g.lambda;
+        for (int row = off; row < H - off; ++row) {
+            for (int col = off; col < W - off; ++col) {
+                const size_t q = (size_t)row * W + col;
+                float xpos = (float)col, ypos = (float)row;
+                const float sv = Sm[q], dv = Dm[q];
+                if (sv - neighbour_max(Sm, W, H, col, row) > 0 && (double)sv > threshd) {
+                    xpos += parabola_peak_cpu(Sm[q - 1], sv, Sm[q + 1]);
+                    ypos += parabola_peak_cpu(Sm[q - W], sv, Sm[q + W]);
+                }
+                if (!(dv - neighbour_max(Dm, W, H, col, row) > 0 && (double)dv > threshd)) continue;
+                xpos += parabola_peak_cpu(Dm[q - 1], dv, Dm[q + 1]);
+                ypos += parabola_peak_cpu(Dm[q - W], dv, Dm[q + W]);
+                if (total < cap) {
+                    bimp_keypoint& kp = out[total];
+                    kp.x = xpos * g.pyrstep;
+                    kp.y = ypos * g.pyrstep;
+                    kp.size = g.lambda * g.pyrstep;
+                    kp.angle = -1;
+                    kp.response = dv;
+                    kp.octave = (int)(g.lambda * g.pyrstep);
+                    kp.class_id = -1;
+                }
+                ++total;
+            }
+        }
+    }
+    *count = total;
+    if (total > cap) {
+        c->err = "keypoints (" + std::to_string(total) + ") exceed the output capacity (" +
+                 std::to_string(cap) + ")";
+        return BIMP_W_OVERFLOW;
+    }
+    return BIMP_OK;
+}
+
 int bimp_bgr_to_gray(const uint8_t* bgr, size_t bgr_pitch, int width, int height, int bgr_on_device,
                      uint8_t* d_gray, size_t gray_pitch, bimp_stream_t stream) {
     if (!bgr || !d_gray || width <= 0 || height <= 0 || bgr_pitch < (size_t)3 * width ||

@@ -181,10 +181,13 @@ public:
         return out;
     }
 
+    // Context::setCPUMode (bimp.hpp:43-44): keypoints from the reference's CPU extraction rules
+    // (bimp_keypoints_cpu_mode) instead of the device NMS.
     void setCPUMode(bool cpu_mode) {
-        if (cpu_mode) throw std::runtime_error("bimp: the CPU-NMS mode is not provided");
+        if (cpu_mode != cpu_mode_) has_changed_keypoints = true;
+        cpu_mode_ = cpu_mode;
     }
-    bool getCPUMode() const { return false; }
+    bool getCPUMode() const { return cpu_mode_; }
 
 #ifdef BIMP_HAVE_OPENCV
     explicit Context(const cv::Mat& gray, std::vector<float> lambdas = cuda::makeLambdasLog(4, 64, 2))
@@ -204,10 +207,23 @@ public:
 private:
     void refresh() {
         if (!has_changed_keypoints) return;
-        keypoints_ = cuda::getKeypoints(image_.data(), (size_t)width_, cuda_data);
+        if (cpu_mode_) {
+            check(bimp_filter_u8(cuda_data.ctx, image_.data(), (size_t)width_, 0, nullptr),
+                  cuda_data.ctx);
+            check(bimp_cells(cuda_data.ctx, nullptr), cuda_data.ctx);
+            keypoints_.resize((size_t)BIMP_KP_MAX_POINTS);
+            int count = 0;
+            const int st = bimp_keypoints_cpu_mode(cuda_data.ctx, keypoints_.data(),
+                                                   BIMP_KP_MAX_POINTS, &count, nullptr);
+            check(st, cuda_data.ctx);
+            keypoints_.resize((size_t)(count < BIMP_KP_MAX_POINTS ? count : BIMP_KP_MAX_POINTS));
+        } else {
+            keypoints_ = cuda::getKeypoints(image_.data(), (size_t)width_, cuda_data);
+        }
         matrix_ = utils::uploadKeypoints(keypoints_);
         has_changed_keypoints = false;
     }
+    bool cpu_mode_ = false;
     std::vector<float> lambdas_;
     int width_ = 0, height_ = 0;
     std::vector<uint8_t> image_;

@@ -212,6 +212,15 @@ int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t str
 int bimp_synth_frame(int width, int height, int nshapes, uint64_t seed, uint8_t* out,
                      size_t pitch);
 
+/* The reference's CPU keypoint mode (getKeypoints with cpu_mode, cuda_bimp.cu:117-124;
+ * cuda_keypoints cuda_bimp.cu:745-808): after bimp_filter_u8 + bimp_cells (or a detect), D
+ * and S are copied to the host and keypoints are extracted there with that mode's rules
+ * (dilation-based maxima, threshold 0.104*max(D) for both maps, inclusive border off,
+ * S-map offsets added to the D-map position when both peak, class_id -1), in (scale, row,
+ * column) order.  Synchronises the stream. */
+int bimp_keypoints_cpu_mode(bimp_ctx* ctx, bimp_keypoint* out, int cap, int* count,
+                            bimp_stream_t stream);
+
 /* getGPUImage's colour step (cuda_bimp.cu:156-179, cv::cuda::cvtColor COLOR_BGR2GRAY): 8-bit
  * interleaved BGR (host or device) -> 8-bit gray on the device, with OpenCV's Q14 weights.
  * The MIN_SIZE upscale of getGPUImage is not provided. */

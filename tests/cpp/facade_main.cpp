@@ -1,7 +1,8 @@
 // Drives the header-only C++ facade (include/bimp/bimp.hpp) the way a reference caller
 // drives bimp::Context: build a context on a frame, fetch keypoints, read responses.
 // Usage: facade_main <width> <height> <frame.u8> <out.bin>
-//   writes: int32 count, then count keypoint records, then the SURF_CUDA matrix (7 x count)
+//   writes: int32 count, then count keypoint records, then the SURF_CUDA matrix (7 x count),
+//   then int32 count and records of the CPU keypoint mode
 #include <cstdio>
 #include <cmath>
 #include <cstdlib>
@@ -28,6 +29,12 @@ int main(int argc, char** argv) {
         out.write(reinterpret_cast<const char*>(&n), sizeof n);
         out.write(reinterpret_cast<const char*>(kps.data()), (std::streamsize)(n * sizeof(bimp::KeyPoint)));
         out.write(reinterpret_cast<const char*>(mat.data()), (std::streamsize)(7 * (size_t)n * sizeof(float)));
+        // the reference's CPU keypoint mode (Context::setCPUMode)
+        ctx.setCPUMode(true);
+        const std::vector<bimp::KeyPoint> cpu = ctx.getKeypointsAndDownload();
+        const int nc = (int)cpu.size();
+        out.write(reinterpret_cast<const char*>(&nc), sizeof nc);
+        out.write(reinterpret_cast<const char*>(cpu.data()), (std::streamsize)(nc * sizeof(bimp::KeyPoint)));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

@@ -65,6 +65,7 @@ def lib():
         L.bo_cells_scale.argtypes = [P, C.c_int, P, P, P]
         L.bo_nms_scale.argtypes = [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_int)]
         L.bo_detect.argtypes = [P, P, C.c_size_t, P, C.c_int, C.POINTER(C.c_int), P, P]
+        L.bo_nms_cpu_mode.argtypes = [P, C.c_int, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.bo_direct_taps.argtypes = [C.POINTER(Scale), C.c_float, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), P, P]
         L.bo_direct_filter_scale.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
@@ -217,6 +218,14 @@ class Oracle:
         out = np.zeros(cap, KP_DTYPE)
         n = C.c_int(count)
         assert lib().bo_nms_scale(self.c, i, ptr(D), ptr(out), cap, C.byref(n)) == 0
+        return out[:min(n.value, cap)], n.value
+
+    def nms_cpu_mode(self, i, D, S, cap=8000):
+        D = np.ascontiguousarray(D, np.float32)
+        S = np.ascontiguousarray(S, np.float32)
+        out = np.zeros(cap, KP_DTYPE)
+        n = C.c_int(0)
+        assert lib().bo_nms_cpu_mode(self.c, i, ptr(D), ptr(S), ptr(out), cap, C.byref(n)) == 0
         return out[:min(n.value, cap)], n.value
 
     def detect(self, img, cap=8000, want_maps=False):

@@ -31,9 +31,16 @@ def test_context_facade_matches_c_abi(tmp_path):
     raw = np.fromfile(out, np.uint8)
     n = int(raw[:4].view(np.int32)[0])
     kps = raw[4:4 + 28 * n].view(B.KEYPOINT_DTYPE)
-    mat = raw[4 + 28 * n:].view(np.float32).reshape(7, n)
+    o = 4 + 28 * n
+    mat = raw[o:o + 28 * n].view(np.float32).reshape(7, n)
+    o += 28 * n
+    nc = int(raw[o:o + 4].view(np.int32)[0])
+    cpu = raw[o + 4:o + 4 + 28 * nc].view(B.KEYPOINT_DTYPE)
     cs = B.CudaStuff(w, h, O.BASE_BANK)
     want, m = cs.detect(img)
     assert n == m
     assert np.array_equal(bits(kps), bits(want))
     assert np.array_equal(bits(B.matrix_to_keypoints(mat)), bits(want))
+    want_cpu, mc = cs.keypoints_cpu_mode()
+    assert nc == mc and nc > 0
+    assert np.array_equal(bits(cpu), bits(want_cpu))

@@ -32,3 +32,24 @@ def test_channel_order_and_rounding(B):
     g = B.bgr_to_gray(bgr)[0]
     assert list(g[:3]) == [29, 150, 76]
     assert g[3] == (1868 * 10 + 9617 * 20 + 4899 * 30 + 8192) >> 14
+
+
+@pytest.mark.parametrize("w,h", [(160, 120), (320, 240)])
+def test_cpu_keypoint_mode_matches_oracle(B, w, h):
+    """bimp_keypoints_cpu_mode equals the oracle's restatement of the reference's CPU mode on
+    the same D and S maps, bit for bit."""
+    import oracle_lib as O
+    from kpmatch import bits
+    cs = B.CudaStuff(w, h, O.BASE_BANK)
+    img = O.synth_frame(w, h, 20, 6)
+    cs.detect(img)
+    got, n = cs.keypoints_cpu_mode()
+    orc = O.Oracle(w, h, O.BASE_BANK)
+    want, m = [], 0
+    for i in range(len(cs.ks)):
+        k, c = orc.nms_cpu_mode(i, cs.get_map(i, B.MAP_DOUBLE), cs.get_map(i, B.MAP_SINGLE))
+        want.append(k)
+        m += c
+    want = np.concatenate(want)
+    assert n == m and n > 0
+    assert np.array_equal(bits(got), bits(want))

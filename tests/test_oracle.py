@@ -329,3 +329,60 @@ def test_cfg0_oracle_keypoints_fixture():
     assert n == len(want)
     got = np.ascontiguousarray(kps).view(np.float32).reshape(n, 7)
     assert np.abs(got - want).max() <= 1e-3
+
+
+# ---------------------------------------------------------------- CPU keypoint mode (Q18)
+def _cpu_mode_maps(w=40, h=36):
+    orc = O.Oracle(w, h, [4.0])
+    s = orc.scales[0]
+    return orc, s, np.zeros((h, w), np.float32), np.zeros((h, w), np.float32)
+
+
+def test_cpu_mode_inclusive_border_and_single_keypoint():
+    orc, s, D, S = _cpu_mode_maps()
+    off = s.off  # 4
+    # samples of 10 - (t - 0.25)^2 in x and 10 - (t + 0.5)^2 in y at t = -1, 0, 1 (exact in
+    # float): the parabola fit puts the peak at +0.25 and -0.5
+    D[off, off] = 9.9375        # on the inclusive border: kept (the GPU NMS needs off < x)
+    D[off, off - 1] = 8.4375
+    D[off, off + 1] = 9.4375
+    D[off - 1, off] = 9.75 - 0.0  # 10 - 0.25
+    D[off + 1, off] = 7.75        # 10 - 2.25
+    D[off, off] = 9.9375
+    kps, n = orc.nms_cpu_mode(0, D, S)
+    assert n == 1
+    k = kps[0]
+    assert k["x"] == off + 0.25
+    # y samples: 9.75, 9.9375 (centre), 7.75 are not one parabola with the centre of x; the fit
+    # of (9.75, 9.9375, 7.75): b = -1, a = -1.1875 -> peak -0.42105...
+    assert abs(k["y"] - (off - 1.0 / 2.375)) < 1e-6
+    assert k["class_id"] == -1 and k["octave"] == 4 and k["size"] == 4.0
+    assert k["response"] == np.float32(9.9375)
+    gkps, gn = orc.nms(0, D)
+    assert gn == 0  # the GPU rule excludes x == off
+
+
+def test_cpu_mode_adds_single_stopped_offsets_and_uses_the_double_threshold():
+    orc, s, D, S = _cpu_mode_maps()
+    y, x = 15, 20
+    D[y, x], D[y, x - 1], D[y, x + 1] = 10.0, 2.0, 6.0
+    S[y, x], S[y, x - 1], S[y, x + 1] = 2.0, 1.0, 0.5   # S peak above 0.104 * max(D) only
+    S[y - 1, x], S[y + 1, x] = 0.25, 0.75
+    kps, n = orc.nms_cpu_mode(0, D, S)
+    assert n == 1
+    kd, _ = orc.nms_cpu_mode(0, D, np.zeros_like(S))
+    # with the S peak the position moves by the S-map parabola offsets as well
+    assert kps[0]["x"] != kd[0]["x"] and kps[0]["y"] != kd[0]["y"]
+    # an S peak below 0.104 * max(D) does not move it
+    S2 = S.copy()
+    S2[y, x] = 1.03
+    kps2, _ = orc.nms_cpu_mode(0, D, S2)
+    assert kps2[0]["x"] == kd[0]["x"] and kps2[0]["y"] == kd[0]["y"]
+
+
+def test_cpu_mode_plateau_and_outside_neighbours():
+    orc, s, D, S = _cpu_mode_maps()
+    D[10, 10] = D[10, 11] = 5.0   # plateau: neither is a strict maximum
+    D[20, 20] = 5.0
+    kps, n = orc.nms_cpu_mode(0, D, S)
+    assert n == 1 and kps[0]["response"] == 5.0
