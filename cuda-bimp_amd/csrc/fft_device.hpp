@@ -255,19 +255,18 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
     }
 }
 
-// LDS floats one group's exchange region needs.  After the first stage lane j writes its R
-// outputs at j*R + r: with R = 16 that stride would put 32 lanes on 2 banks, so that exchange
-// stores element i at i + i/R (an odd lane stride R + 1; the offsets of one lane and of the
-// reads' r-steps stay compile-time constants, so the accesses still pair up).  Later exchanges
-// write runs of >= R consecutive lanes and keep the plain layout.
+// LDS floats one group's exchange region needs, and the address of element i in it.  After
+// the first stage lane j writes its R outputs at j*R + r: with R = 16 that stride would put 32
+// lanes on 2 banks.  That exchange XOR-swizzles the bank bits of each 32-word row with the row
+// index (a bijection within the row), which spreads those writes over all banks and keeps the
+// reads (32 consecutive elements of one row per access) conflict-free, at no extra LDS.
+// Later exchanges write runs of >= R consecutive lanes and keep the plain layout.
 template <int N>
-__host__ __device__ constexpr int fft_lds_floats() {
-    return N + N / RadInfo<typename FFTCfg<N>::Rad>::first;
-}
+__host__ __device__ constexpr int fft_lds_floats() { return N; }
 template <int Ns, int R>
 __device__ __forceinline__ int lds_pad(int i) {
     if constexpr (Ns == 1)
-        return i + i / R;
+        return (i & ~31) | ((i ^ (i >> 5)) & 31);
     else
         return i;
 }

@@ -112,7 +112,9 @@ struct ColCfg {
     static constexpr int TPV = FFTCfg<N>::TPV;
     static constexpr int CB = TPV == 64 ? (N <= 2048 ? 8 : 4) : (512 / TPV > 16 ? 16 : 512 / TPV);
     static constexpr int THREADS = CB * TPV;
-    static constexpr int S = N + 8;
+    // column stride (complex): 2 (mod 16) puts the 8 columns of a 16-lane access 4 banks
+    // apart, so staging writes (column index fastest) do not collide
+    static constexpr int S = N + 2;
 };
 
 // Column FFT of the row spectra R (H rows, ldr complex per row) with clamp padding in y:
@@ -169,7 +171,7 @@ struct InvColCfg {
     // lane + TPV*(e/RL) + (e%RL)*BR, so block membership is known at compile time
     static constexpr int RL = RadInfo<typename FFTCfg<FH>::Rad>::last;
     static constexpr int BR = FH / RL;
-    static constexpr int SS = BR + 8;  // staging stride (float2); == 8 (mod 32) when 32 | BR
+    static constexpr int SS = BR + 2;  // staging stride (float2); 2 (mod 16): conflict-free reads
     static constexpr size_t STAGE = (size_t)CB * SS * sizeof(float2);
     static constexpr size_t EXCH = (size_t)CB * fft_lds_floats<FH>() * sizeof(float);
     static constexpr size_t LDS = STAGE > EXCH ? STAGE : EXCH;
