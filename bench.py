@@ -291,6 +291,21 @@ def load_traffic(kernel):
         return None
 
 
+def batch_form_child(timeout=600):
+    """Runs the batching form in a child process (`bench.py --only-batch`), so that a failure
+    there is reported in its entry instead of ending the benchmark."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--only-batch"],
+                           stdout=subprocess.PIPE, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"batch form exceeded {timeout} s"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"batch form child exited with status {r.returncode}"}
+    return json.loads(lines[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,6 +317,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the BF16/FP16 precision-form lines and the FP64 oracle check")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form line")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batching form and print its entry as one JSON line")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form line")
     ap.add_argument("--form", choices=["plain", "bf16", "fp16"], default="plain",
                     help="path timed for the main line (plain = FFT-FP32, the headline)")
@@ -320,6 +337,9 @@ def main():
     if B.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.only_batch:
+        print(json.dumps(batch_form(B, B.Stream())), flush=True)
+        return
     img = B.synth_frame(W, H, NSHAPES, SEED)
     opt = B.default_options()
     if args.form != "plain":
@@ -411,8 +431,9 @@ def main():
                   flush=True)
         out["precision_form"] = forms
     if not args.no_batch and args.form == "plain":
-        out["batch_form"] = batch_form(B, stream)
-        print(f"batch form: {out['batch_form']['mpix_per_s']} MPix/s", file=sys.stderr, flush=True)
+        out["batch_form"] = batch_form_child()
+        print(f"batch form: {out['batch_form'].get('mpix_per_s')} MPix/s", file=sys.stderr,
+              flush=True)
     if not args.no_diff and args.form == "plain":
         out["diff_form"] = diff_form(B)
         print(f"diff form: backward {out['diff_form']['backward_ms_per_image']} ms/image",

@@ -119,8 +119,10 @@ int capture(bimp_batch* b, hipGraphExec_t* exec, hipGraph_t* keep, F body) {
     return BIMP_OK;
 }
 
-// The batch graph: fork on the capture stream, one branch per context, join.
-int build(bimp_batch* b) {
+// The batch graph: fork on the capture stream, one branch per context, join.  With
+// `per_frame`, also one graph per frame (the latency of a frame alone); the tuner's trial
+// plans skip those.
+int build(bimp_batch* b, bool per_frame = true) {
     drop_graphs(b);
     std::vector<std::pair<bimp_ctx*, std::vector<int>>> branches;
     for (auto& c : b->cls) {
@@ -158,7 +160,7 @@ int build(bimp_batch* b) {
         }
         return BIMP_OK;
     });
-    if (st) return st;
+    if (st || !per_frame) return st;
     b->frameExec.assign(b->nframes, nullptr);
     for (int f = 0; f < b->nframes; ++f) {
         bimp_ctx* x = b->cls[b->frameClass[f]].ctx[0];
@@ -186,41 +188,49 @@ int time_batch(bimp_batch* b, int iters, hipStream_t s, float* ms) {
 }
 
 // Greedy plan search: for each frame size (smallest first) add contexts while the batch gets
-// faster by more than 2%, then try the smallest inverse-pass sub-batch.
+// faster by more than 2%, then try the smallest inverse-pass sub-batch.  Contexts a trial
+// rejects are released only once the final graphs exist, so no instantiated graph ever refers
+// to freed memory.
 int tune(bimp_batch* b, hipStream_t s) {
-    float best = 0;
-    int st = build(b);
-    if (st) return st;
-    if ((st = time_batch(b, 3, s, &best))) return st;
-    std::vector<size_t> order(b->cls.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-        return (long)b->cls[x].w * b->cls[x].h < (long)b->cls[y].w * b->cls[y].h;
-    });
-    for (size_t ci : order) {
-        SizeClass& c = b->cls[ci];
-        const size_t maxK = std::min<size_t>(4, c.frames.size());
-        while (c.ctx.size() < maxK) {
-            if ((st = ensure_ctx(b, c, c.ctx.size()))) return st;
-            float t = 0;
-            if ((st = build(b)) || (st = time_batch(b, 3, s, &t))) return st;
-            if (t < 0.98f * best) {
-                best = t;
-                continue;
+    std::vector<bimp_ctx*> rejected;
+    const int st = [&]() -> int {
+        float best = 0;
+        int r = build(b, false);
+        if (r) return r;
+        if ((r = time_batch(b, 3, s, &best))) return r;
+        std::vector<size_t> order(b->cls.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+            return (long)b->cls[x].w * b->cls[x].h < (long)b->cls[y].w * b->cls[y].h;
+        });
+        for (size_t ci : order) {
+            SizeClass& c = b->cls[ci];
+            const size_t maxK = std::min<size_t>(4, c.frames.size());
+            while (c.ctx.size() < maxK) {
+                if ((r = ensure_ctx(b, c, c.ctx.size()))) return r;
+                float t = 0;
+                if ((r = build(b, false)) || (r = time_batch(b, 3, s, &t))) return r;
+                if (t < 0.98f * best) {
+                    best = t;
+                    continue;
+                }
+                rejected.push_back(c.ctx.back());
+                c.ctx.pop_back();
+                break;
             }
-            bimp_destroy(c.ctx.back());
-            c.ctx.pop_back();
-            break;
+            c.invBatch = 1;
+            float t = 0;
+            if ((r = build(b, false)) || (r = time_batch(b, 3, s, &t))) return r;
+            if (t < 0.98f * best)
+                best = t;
+            else
+                c.invBatch = 0;
         }
-        c.invBatch = 1;
-        float t = 0;
-        if ((st = build(b)) || (st = time_batch(b, 3, s, &t))) return st;
-        if (t < 0.98f * best)
-            best = t;
-        else
-            c.invBatch = 0;
-    }
-    return build(b);
+        return build(b, true);
+    }();
+    (void)hipStreamSynchronize(s);
+    for (auto* x : rejected) bimp_destroy(x);
+    return st;
 }
 
 }  // namespace
