@@ -426,7 +426,8 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             d.kpad = s.g.kX;
             d.kW = s.g.kW;
             d.ext = bimph::direct_extent(s.g, c->opt.sigma_ratio, c->opt.direct_sigmas);
-            if (bimpk::direct_geometry(d) <= bimpk::kMaxLds) s.precision = c->opt.precision;
+            if (bimpk::direct_geometry(d) <= bimpk::kMaxLds && d.lwp <= bimpk::kDirMaxLwp)
+                s.precision = c->opt.precision;
         }
         if (s.precision == BIMP_PRECISION_FP32 &&
             (!bimpk::fft_size_supported(s.g.FH) || !bimpk::fft_size_supported(s.g.FW) ||

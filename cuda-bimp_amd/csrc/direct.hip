@@ -93,39 +93,69 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
 
     // Stage rows y0-k .. and columns x0-k .. of Pd as four shifted copies (tile sample c at
     // position c - s of copy s).  Positions past the image window hold zero; they only meet
-    // zero taps.
+    // zero taps.  Wave w takes rows w, w + kDirWaves, ..., lane l columns l, l + 64, l + 128;
+    // the byte loads of kStageRows rows are all issued before any is written, so that their
+    // latencies overlap instead of adding up row by row.
     const bool inside = y0 - k >= 0 && y0 + TY + k <= ds.H && x0 - k >= 0 && x0 + TX + k <= ds.W;
-    for (int r = wave; r < ds.LH; r += kDirWaves) {
-        const bool rowin = r < TY + 2 * k;
-        const int gy = !rowin ? 0 : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
-        const uint8_t* row = img + (size_t)gy * pitch;
-        T* dst = copies + r * LWP;
-        for (int c = lane; c < LWP; c += 64) {
-            T v = T(0.f);
-            if (rowin && c < TWi) {
-                const int gx = inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k);
-                v = T((float)row[gx]);
+    {
+        constexpr int kStageRows = 4, kCols = kDirMaxLwp / 64;
+        for (int r0 = wave; r0 < ds.LH; r0 += kStageRows * kDirWaves) {
+            float v[kStageRows][kCols];
+#pragma unroll
+            for (int q = 0; q < kStageRows; ++q) {
+                const int r = r0 + q * kDirWaves;
+                const bool rowin = r < TY + 2 * k;
+                const int gy = !rowin ? 0
+                                      : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
+                const uint8_t* row = img + (size_t)gy * pitch;
+#pragma unroll
+                for (int i = 0; i < kCols; ++i) {
+                    const int c = lane + 64 * i;
+                    float x = 0.f;
+                    if (rowin && c < TWi) {
+                        const int gx = inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k);
+                        x = (float)row[gx];
+                    }
+                    v[q][i] = x;
+                }
             }
 #pragma unroll
-            for (int sft = 0; sft < 4; ++sft)
-                if (c >= sft) dst[sft * csize + c - sft] = v;
+            for (int q = 0; q < kStageRows; ++q) {
+                const int r = r0 + q * kDirWaves;
+                if (r >= ds.LH) break;
+                T* dst = copies + r * LWP;
+#pragma unroll
+                for (int i = 0; i < kCols; ++i) {
+                    const int c = lane + 64 * i;
+                    if (c >= LWP) break;
+                    const T tv = T(v[q][i]);
+#pragma unroll
+                    for (int sft = 0; sft < 4; ++sft)
+                        if (c >= sft) dst[sft * csize + c - sft] = tv;
+                }
+            }
         }
     }
     __syncthreads();
 
     // kW x kW box sums of the tile's output pixels: vertical running sums over kW rows into
-    // vs[r][c] (two segments of TY/2 rows per column), then horizontal ones, stored back into
-    // vs as a TY x TX array.
-    if (tid < 2 * TWi) {
-        const int seg = tid >= TWi ? 1 : 0;
-        const int c = tid - seg * TWi, r0 = seg * (TY / 2);
-        const T* col = copies + c;
-        float sum = 0.f;
-        for (int u = 0; u < kW; ++u) sum += (float)col[(r0 + u) * LWP];
-        vs[r0 * ds.vsp + c] = sum;
-        for (int r = 1; r < TY / 2; ++r) {
-            sum += (float)col[(r0 + r + kW - 1) * LWP] - (float)col[(r0 + r - 1) * LWP];
-            vs[(r0 + r) * ds.vsp + c] = sum;
+    // vs[r][c] (two segments of TY/2 rows per column when the workgroup has a thread for each,
+    // else one of TY rows), then horizontal ones, stored back into vs as a TY x TX array.
+    {
+        constexpr int NT = kDirWaves * 64;
+        const int nseg = 2 * TWi <= NT ? 2 : 1;
+        const int rows = TY / nseg;
+        for (int t = tid; t < nseg * TWi; t += NT) {
+            const int seg = t >= TWi ? 1 : 0;
+            const int c = t - seg * TWi, r0 = seg * rows;
+            const T* col = copies + c;
+            float sum = 0.f;
+            for (int u = 0; u < kW; ++u) sum += (float)col[(r0 + u) * LWP];
+            vs[r0 * ds.vsp + c] = sum;
+            for (int r = 1; r < rows; ++r) {
+                sum += (float)col[(r0 + r + kW - 1) * LWP] - (float)col[(r0 + r - 1) * LWP];
+                vs[(r0 + r) * ds.vsp + c] = sum;
+            }
         }
     }
     __syncthreads();
@@ -280,8 +310,8 @@ size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const*
         const size_t l = direct_geometry(d);
         if (l > kMaxLds || d.off != ds[i].off || d.nvb != ds[i].nvb || d.nrho != ds[i].nrho ||
             d.LH != ds[i].LH || d.lwp != ds[i].lwp || d.vsp != ds[i].vsp ||
-            d.cstride != ds[i].cstride || 2 * (kDirTX + 2 * ds[i].kpad) > kDirWaves * 64 ||
-            ds[i].kW != 2 * ds[i].kpad + 1 || ds[i].off < 0)
+            d.cstride != ds[i].cstride || ds[i].kW != 2 * ds[i].kpad + 1 || ds[i].off < 0 ||
+            d.lwp > kDirMaxLwp)
             return 0;
         lds = std::max(lds, l);
         jobs[k].ds = ds[i];

@@ -45,6 +45,10 @@ constexpr size_t kMaxLds = 160 * 1024 - 1024;
 // Workgroup tile: kDirWaves wavefronts x kDirRows output rows, kDirGroups x 16 columns.
 constexpr int kDirRows = 4, kDirGroups = 8, kDirWaves = 8;
 constexpr int kDirTY = kDirWaves * kDirRows, kDirTX = kDirGroups * 16;
+// widest staged image row (elements): tile columns plus the kernel's 55-tap support, rounded
+constexpr int kDirMaxLwp = 192;
+// the horizontal box-sum pass gives each thread 8 columns of one output row
+static_assert(kDirWaves * 64 == kDirTY * (kDirTX / 8), "one thread per 8 tile columns");
 struct DirectScale {
     int H, W, FH, FW;        // data size, transform size (the periodic domain)
     int kpad, kW;            // kernel centre kX (clamp padding, box half-width) and width
