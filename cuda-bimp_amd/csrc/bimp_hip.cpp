@@ -372,7 +372,7 @@ void bimp_default_options(bimp_options* o) {
     o->input_scale = (float)(255.0 / 65536.0);
     o->flags = 0;
     o->precision = BIMP_PRECISION_FP32;
-    o->direct_max_lambda = 8.0f;
+    o->direct_max_lambda = 64.0f;  // every scale of the reference's banks (4 .. 64)
     o->direct_sigmas = 0.f;
 }
 

@@ -60,8 +60,14 @@ struct Mfma<_Float16> {
 // Row (or column) of the periodic padded image (padDataClampToBorder rule,
 // convolutionFFT2D.cuh:56-85): index i of the F-periodic domain, data size n, border k.
 __device__ __forceinline__ int pad_index(int i, int n, int F, int k) {
-    i %= F;
-    if (i < 0) i += F;
+    if (i < 0 && i >= -F)
+        i += F;
+    else if (i >= F && i < 2 * F)
+        i -= F;
+    else if (i < 0 || i >= F) {
+        i %= F;
+        if (i < 0) i += F;
+    }
     return i < n ? i : (i < n + k ? n - 1 : 0);
 }
 
@@ -99,6 +105,13 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     const bool inside = y0 - k >= 0 && y0 + TY + k <= ds.H && x0 - k >= 0 && x0 + TX + k <= ds.W;
     {
         constexpr int kStageRows = 4, kCols = kDirMaxLwp / 64;
+        // source column of staged column lane + 64 i (-1: zero), the same for every row
+        int gxs[kCols];
+#pragma unroll
+        for (int i = 0; i < kCols; ++i) {
+            const int c = lane + 64 * i;
+            gxs[i] = c >= TWi ? -1 : (inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k));
+        }
         for (int r0 = wave; r0 < ds.LH; r0 += kStageRows * kDirWaves) {
             float v[kStageRows][kCols];
 #pragma unroll
@@ -109,15 +122,8 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
                                       : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
                 const uint8_t* row = img + (size_t)gy * pitch;
 #pragma unroll
-                for (int i = 0; i < kCols; ++i) {
-                    const int c = lane + 64 * i;
-                    float x = 0.f;
-                    if (rowin && c < TWi) {
-                        const int gx = inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k);
-                        x = (float)row[gx];
-                    }
-                    v[q][i] = x;
-                }
+                for (int i = 0; i < kCols; ++i)
+                    v[q][i] = (rowin && gxs[i] >= 0) ? (float)row[gxs[i]] : 0.f;
             }
 #pragma unroll
             for (int q = 0; q < kStageRows; ++q) {
@@ -149,9 +155,17 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
             const int seg = t >= TWi ? 1 : 0;
             const int c = t - seg * TWi, r0 = seg * rows;
             const T* col = copies + c;
-            float sum = 0.f;
-            for (int u = 0; u < kW; ++u) sum += (float)col[(r0 + u) * LWP];
+            // integer-valued partial sums below 2^24: exact in FP32 in any order
+            float s4[4] = {0.f, 0.f, 0.f, 0.f};
+            int u = 0;
+            for (; u + 4 <= kW; u += 4) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) s4[i] += (float)col[(r0 + u + i) * LWP];
+            }
+            for (; u < kW; ++u) s4[0] += (float)col[(r0 + u) * LWP];
+            float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
             vs[r0 * ds.vsp + c] = sum;
+#pragma unroll 4
             for (int r = 1; r < rows; ++r) {
                 sum += (float)col[(r0 + r + kW - 1) * LWP] - (float)col[(r0 + r - 1) * LWP];
                 vs[(r0 + r) * ds.vsp + c] = sum;
@@ -163,8 +177,14 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     {
         const int r = tid >> 4, xs = (tid & 15) * 8;
         const float* vr = vs + r * ds.vsp + xs;
-        float sum = 0.f;
-        for (int v = 0; v < kW; ++v) sum += vr[v];
+        float s4[4] = {0.f, 0.f, 0.f, 0.f};
+        int v = 0;
+        for (; v + 4 <= kW; v += 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s4[i] += vr[v + i];
+        }
+        for (; v < kW; ++v) s4[0] += vr[v];
+        float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         bsum[0] = sum;
 #pragma unroll
         for (int i = 1; i < 8; ++i) {
@@ -240,25 +260,29 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
 #pragma unroll
         for (int g = 0; g < NCG; ++g) {
             const int xl = 16 * g + 4 * kb, x = x0 + xl;
-            const float4 bx = *reinterpret_cast<const float4*>(vs + yl * TX + xl);
-            const float bxs[4] = {bx.x, bx.y, bx.z, bx.w};
-            float c4[4];
+            // the lane pair (even filter, odd filter) splits the four pixels: the even lane
+            // finishes pixels 0-1, the odd lane pixels 2-3, after swapping the other halves
+            const int tb = (jf & 1) ? 2 : 0;
+            const float2 bx = *reinterpret_cast<const float2*>(vs + yl * TX + xl + tb);
+            const float bxs[2] = {bx.x, bx.y};
+            float c2[2];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const float mine = acc[j][g][t];
-                const float other = __shfl_xor(mine, 1, 64);
-                const float e = ds.fscale * ((jf & 1) ? other : mine) - dce * bxs[t];
-                const float od = ds.fscale * ((jf & 1) ? mine : other) - dco * bxs[t];
-                c4[t] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
+            for (int q = 0; q < 2; ++q) {
+                const float mine = (jf & 1) ? acc[j][g][2 + q] : acc[j][g][q];
+                const float send = (jf & 1) ? acc[j][g][q] : acc[j][g][2 + q];
+                const float other = __shfl_xor(send, 1, 64);
+                const float e = ds.fscale * ((jf & 1) ? other : mine) - dce * bxs[q];
+                const float od = ds.fscale * ((jf & 1) ? mine : other) - dco * bxs[q];
+                c2[q] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
             }
-            if (!(jf & 1) && y < ds.Hc) {
-                float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x;
-                if (x + 3 < ds.Wc) {
-                    *reinterpret_cast<float4*>(out) = make_float4(c4[0], c4[1], c4[2], c4[3]);
+            if (y < ds.Hc) {
+                float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x + tb;
+                if (x + tb + 1 < ds.Wc) {
+                    *reinterpret_cast<float2*>(out) = make_float2(c2[0], c2[1]);
                 } else {
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        if (x + t < ds.Wc) out[t] = c4[t];
+                    for (int q = 0; q < 2; ++q)
+                        if (x + tb + q < ds.Wc) out[q] = c2[q];
                 }
             }
         }
