@@ -210,12 +210,28 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     for (int j = 0; j < M; ++j)
 #pragma unroll
         for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // Software-pipelined over the K-steps it = rho * nvb + vb: the image and filter
-    // fragments of step it + 1 are in flight while the MFMAs of step it run.
+    // Software-pipelined over the K-steps it = rho * nvb + vb: the image fragments of step
+    // it + 1 (LDS) and the filter fragments of step it + 2 (L2) are in flight while the MFMAs
+    // of step it run.  Fragment offsets advance incrementally (no division per step).
     const int nvb = ds.nvb, nsteps = ds.nrho * nvb;
-    auto load_a = [&](int it, v8* a) {
-        const int rho = it / nvb, vb = it - rho * nvb;
-        const T* rowp = lanebase + 4 * rho * LWP + 8 * vb;
+    // step cursor: (rho, vb) and the element offsets of its image and filter fragments
+    struct Cursor {
+        int vb, aoff, boff;
+    };
+    // filter fragments of u0 = 4*rho - j sit at row index u0 + M - 1
+    const v8* __restrict__ bbase = bfrag + (size_t)(M - 1) * nvb * 64 + lane;
+    auto advance = [&](Cursor& c) {
+        if (++c.vb == nvb) {
+            c.vb = 0;
+            c.aoff += 4 * LWP - 8 * (nvb - 1);
+            c.boff += (3 * nvb + 1) * 64;
+        } else {
+            c.aoff += 8;
+            c.boff += 64;
+        }
+    };
+    auto load_a = [&](const Cursor& c, v8* a) {
+        const T* rowp = lanebase + c.aoff;
 #pragma unroll
         for (int g = 0; g < NCG; ++g) {
             const v4 lo = *reinterpret_cast<const v4*>(rowp + 16 * g);
@@ -223,22 +239,25 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
             a[g] = v8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
         }
     };
-    auto load_b = [&](int it, v8* b) {
-        const int rho = it / nvb, vb = it - rho * nvb;
-        // filter fragments of u0 = 4*rho - j sit at row index u0 + M - 1
-        const v8* bro = bfrag + ((size_t)(4 * rho + M - 1) * nvb + vb) * 64 + lane;
+    auto load_b = [&](const Cursor& c, v8* b) {
+        const v8* bro = bbase + c.boff;
 #pragma unroll
         for (int j = 0; j < M; ++j) b[j] = bro[-j * nvb * 64];
     };
-    v8 a[NCG], b[M];
-    load_a(0, a);
-    load_b(0, b);
+    v8 a[NCG], b[M], b1[M];
+    Cursor ca{0, 0, 0};
+    load_a(ca, a);
+    load_b(ca, b);
+    Cursor cb = ca;
+    if (nsteps > 1) advance(cb);
+    load_b(cb, b1);
+    if (nsteps > 1) advance(ca);  // ca: step it + 1, cb: step it + 2 (both clamped)
+    if (nsteps > 2) advance(cb);
 #pragma unroll 1
     for (int it = 0; it < nsteps; ++it) {
         v8 an[NCG], bn[M];
-        const int nx = it + 1 < nsteps ? it + 1 : it;
-        load_a(nx, an);
-        load_b(nx, bn);
+        load_a(ca, an);
+        load_b(cb, bn);
 #pragma unroll
         for (int j = 0; j < M; ++j)
 #pragma unroll
@@ -246,7 +265,12 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
 #pragma unroll
         for (int g = 0; g < NCG; ++g) a[g] = an[g];
 #pragma unroll
-        for (int j = 0; j < M; ++j) b[j] = bn[j];
+        for (int j = 0; j < M; ++j) {
+            b[j] = b1[j];
+            b1[j] = bn[j];
+        }
+        if (it + 2 < nsteps) advance(ca);
+        if (it + 3 < nsteps) advance(cb);
     }
     __syncthreads();  // the box sums stored before the main loop are read below
 
