@@ -129,8 +129,9 @@ int launch_cells_scale(bimp_ctx* c, size_t i, hipStream_t s) {
 // while their T stays within it, so that T is consumed from the 256 MB last-level cache.
 constexpr size_t kTBudgetBytes = 160u << 20;
 
-// Direct-form scales of one level per launch (fills the chip's tail with the lighter scales).
-constexpr int kDirectLaunchScales = 4;
+// Direct-form scales per launch: consecutive direct scales of every level share one launch,
+// so the lighter ones fill the tail of the heavier ones.
+constexpr int kDirectLaunchScales = 8;
 
 int inverse_batch(size_t tstride_elems) {
     const size_t per = tstride_elems * sizeof(float2);
@@ -184,7 +185,12 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                     maxext = std::max(maxext, sk.ds.ext);
                 }
                 LaunchScope ls(c, s, F_DIRECT, bytes, maxext);
-                HIPCHK(c, bimpk::launch_direct(lv[g.level], lp[g.level], st.jobs, st.n, st.tiles,
+                bimpk::DirectLevels dl{};
+                for (int l = 0; l <= c->maxLevel && l < bimpk::kDirMaxLevels; ++l) {
+                    dl.img[l] = lv[l];
+                    dl.pitch[l] = lp[l];
+                }
+                HIPCHK(c, bimpk::launch_direct(dl, st.jobs, st.n, st.tiles,
                                                st.lds, s0.precision == BIMP_PRECISION_FP16, s));
             }
             if (cells) {
@@ -481,10 +487,10 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     for (int i = 0; i < (int)c->sc.size(); ++i) {
         const auto& g = c->sc[i].g;
         if (c->sc[i].precision != BIMP_PRECISION_FP32) {
-            // consecutive direct scales of one level share a launch
+            // consecutive direct scales share a launch
             if (!c->steps.empty() && c->steps.back().direct && !prevFft) {
                 Step& last = c->steps.back();
-                if (c->sc[last.index].g.level == g.level && last.n < kDirectLaunchScales) {
+                if (last.n < kDirectLaunchScales) {
                     ++last.n;
                     continue;
                 }
@@ -580,14 +586,17 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             std::vector<bimpk::DirectScale> ds;
             std::vector<const void*> fr;
             std::vector<float*> cm;
+            std::vector<int> lvl;
             for (int k = 0; k < st.n; ++k) {
                 const Scale& sk = c->sc[st.index + k];
                 ds.push_back(sk.ds);
                 fr.push_back(sk.frag);
                 cm.push_back(sk.C);
+                lvl.push_back(sk.g.level);
             }
             std::vector<bimpk::DirectJob> jobs(st.n);
-            st.lds = bimpk::direct_jobs(ds.data(), fr.data(), cm.data(), st.n, jobs.data(), &st.tiles);
+            st.lds = bimpk::direct_jobs(ds.data(), fr.data(), cm.data(), lvl.data(), st.n,
+                                        jobs.data(), &st.tiles);
             if (!st.lds) return bail(BIMP_E_UNSUPPORTED, "direct-form job table");
             if (dalloc(&st.jobs, jobs.size()) != hipSuccess) return bail(BIMP_E_ALLOC, "direct jobs");
             if (hipMemcpy(st.jobs, jobs.data(), jobs.size() * sizeof(bimpk::DirectJob),

@@ -75,14 +75,15 @@ __device__ __forceinline__ int pad_index(int i, int n, int F, int k) {
 // workgroup b works on the scale whose tile range [tile0, tile0 + tiles) holds b.
 template <class T>
 __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
-        const uint8_t* __restrict__ img, size_t pitch, const DirectJob* __restrict__ jobs,
-        int njobs) {
+        const DirectLevels lv, const DirectJob* __restrict__ jobs, int njobs) {
     typedef typename Mfma<T>::v8 v8;
     typedef typename Mfma<T>::v4 v4;
     constexpr int M = kDirRows, NCG = kDirGroups, TY = kDirTY, TX = kDirTX;
     int js = njobs - 1;
     while (js > 0 && (int)blockIdx.x < jobs[js].tile0) --js;  // uniform
     const DirectJob& job = jobs[js];
+    const uint8_t* __restrict__ img = lv.img[job.level];
+    const size_t pitch = lv.pitch[job.level];
     const DirectScale& ds = job.ds;
     const v8* __restrict__ bfrag = reinterpret_cast<const v8*>(job.frag);
     float* __restrict__ Cm = job.C;
@@ -342,8 +343,8 @@ hipError_t prepare_direct() {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
 }
 
-size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const* C, int n,
-                   DirectJob* jobs, int* total_tiles) {
+size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const* C,
+                   const int* level, int n, DirectJob* jobs, int* total_tiles) {
     // heaviest scales first, so that the short ones fill the tail of the launch
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
@@ -365,6 +366,8 @@ size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const*
         jobs[k].ds = ds[i];
         jobs[k].frag = frag[i];
         jobs[k].C = C[i];
+        jobs[k].level = level[i];
+        if (level[i] < 0 || level[i] >= kDirMaxLevels) return 0;
         jobs[k].tiles_x = (ds[i].Wc + kDirTX - 1) / kDirTX;
         jobs[k].tile0 = t;
         t += jobs[k].tiles_x * ((ds[i].Hc + kDirTY - 1) / kDirTY);
@@ -373,13 +376,13 @@ size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const*
     return lds;
 }
 
-hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectJob* d_jobs, int njobs,
+hipError_t launch_direct(const DirectLevels& lv, const DirectJob* d_jobs, int njobs,
                          int total_tiles, size_t lds, bool fp16, hipStream_t s) {
     if (lds > kMaxLds || njobs < 1 || total_tiles < 1) return hipErrorInvalidValue;
     if (fp16)
-        k_direct<_Float16><<<total_tiles, kDirWaves * 64, lds, s>>>(img, pitch, d_jobs, njobs);
+        k_direct<_Float16><<<total_tiles, kDirWaves * 64, lds, s>>>(lv, d_jobs, njobs);
     else
-        k_direct<__bf16><<<total_tiles, kDirWaves * 64, lds, s>>>(img, pitch, d_jobs, njobs);
+        k_direct<__bf16><<<total_tiles, kDirWaves * 64, lds, s>>>(lv, d_jobs, njobs);
     return hipGetLastError();
 }
 

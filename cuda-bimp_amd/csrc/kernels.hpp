@@ -60,22 +60,30 @@ struct DirectScale {
     float dc[2 * kNumOri];   // box coefficients, filter 2o even, 2o+1 odd
 };
 
-// One scale of a direct launch: its geometry, filter fragments, output maps and tile range.
+// One scale of a direct launch: its geometry, filter fragments, output maps, tile range and
+// pyramid level.
 struct DirectJob {
     DirectScale ds;
     const void* frag;
     float* C;
     int tiles_x, tile0;
+    int level;
+};
+// The pyramid levels a direct launch reads (level 0 is the caller's frame).
+constexpr int kDirMaxLevels = 8;
+struct DirectLevels {
+    const uint8_t* img[kDirMaxLevels];
+    size_t pitch[kDirMaxLevels];
 };
 
 // Fills off/nvb/nrho/LH/lwp/vsp/cstride from kpad, kW, ext; returns the dynamic LDS bytes.
 size_t direct_geometry(DirectScale& ds);
-// Job table of n scales of one level, heaviest first; returns the launch's LDS bytes (0 if a
-// scale does not fit) and the total tile count.
-size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const* C, int n,
-                   DirectJob* jobs, int* total_tiles);
+// Job table of n scales (any pyramid levels), heaviest first; returns the launch's LDS bytes
+// (0 if a scale does not fit) and the total tile count.
+size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const* C,
+                   const int* level, int n, DirectJob* jobs, int* total_tiles);
 hipError_t prepare_direct();
-hipError_t launch_direct(const uint8_t* img, size_t pitch, const DirectJob* d_jobs, int njobs,
+hipError_t launch_direct(const DirectLevels& lv, const DirectJob* d_jobs, int njobs,
                          int total_tiles, size_t lds, bool fp16, hipStream_t s);
 
 constexpr int kNmsThreads = 256;
