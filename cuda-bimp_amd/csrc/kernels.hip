@@ -179,29 +179,33 @@ struct InvColCfg {
 
 template <int FH>
 __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
-        ScaleSet ss, int FW, int Hc, int nblk, float2* __restrict__ Tbase, size_t tstride,
-        const float2* __restrict__ tw) {
+        ScaleSet ss, int FW, int Hc, int nblk, int units, float2* __restrict__ Tbase,
+        size_t tstride, const float2* __restrict__ tw) {
     using C = FFTCfg<FH>;
     using IC = InvColCfg<FH>;
     constexpr int CB = IC::CB;
     extern __shared__ float2 lds2[];
-    // XCD-aware numbering: the i-th workgroup of XCD x handles orientation i % 8 of column
-    // block 8*(i/8) + x (column blocks of all scales of the batch concatenated)
-    const int xcd = blockIdx.x % 8, i = blockIdx.x / 8;
-    const int o = i % kNumOri, gcb = (i / kNumOri) * 8 + xcd;
-    if (gcb >= nblk * ss.n) return;  // uniform per workgroup: no barrier is skipped
-    const int sidx = gcb / nblk, k0 = (gcb % nblk) * CB;
-    const float2* DT = ss.DT[sidx];
-    const float* G = ss.G[sidx];
-    float2* To = Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW;
     const int half = FW / 2;
-    const int og = o <= 4 ? o : kNumOri - o;
     const int g = threadIdx.x / C::TPV, lane0 = threadIdx.x % C::TPV;
     float* my = reinterpret_cast<float*>(lds2) + (size_t)g * fft_lds_floats<FH>();
-
+    // A unit is one pass (direct or partner) of one orientation of one column block.  The
+    // grid is at most one resident round of workgroups and walks the units with stride
+    // gridDim.x (a multiple of 8), so the work splits evenly in half-block units instead of
+    // leaving most of a last round idle.  XCD-aware numbering: unit u runs on XCD u % 8 and
+    // its index i = u / 8 there gives orientation i % 8, pass (i / 8) % 2 and column block
+    // 8 * (i / 16) + u % 8, so the orientations and both passes of a column block stay on one
+    // XCD and share its D and bank columns through that XCD's L2.
 #pragma unroll 1
-    for (int partner = 0; partner < 2; ++partner) {
-        // lane-derived indices are recomputed per pass rather than kept live across passes
+    for (int u = blockIdx.x; u < units; u += gridDim.x) {
+        const int xcd = u % 8, i = u / 8;
+        const int o = i % kNumOri, partner = (i / kNumOri) % 2, gcb = (i / (2 * kNumOri)) * 8 + xcd;
+        if (gcb >= nblk * ss.n) continue;  // uniform per workgroup: no barrier is skipped
+        const int sidx = gcb / nblk, k0 = (gcb % nblk) * CB;
+        const float2* DT = ss.DT[sidx];
+        const float* G = ss.G[sidx];
+        float2* To = Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW;
+        const int og = o <= 4 ? o : kNumOri - o;
+        // lane-derived indices are recomputed per unit rather than kept live across units
         int lane = lane0;
         asm volatile("" : "+v"(lane));
         // direct: column kx = k0 + g; partner: stored column kx = k0 + 1 + g, output FW - kx
@@ -827,6 +831,22 @@ int row_vectors_per_block(int n) {
     return 0;
 }
 
+// Resident workgroups of k_inv_cols<N> on the whole device (one round), a multiple of 8 so
+// that the unit stride keeps each unit on its XCD; computed once per size.
+template <int N>
+static int inv_cols_slots() {
+    static int slots = [] {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_inv_cols<N>, InvColCfg<N>::THREADS,
+                                                         InvColCfg<N>::LDS) != hipSuccess)
+            return 0;
+        return std::max(1, cus * per / 8) * 8;
+    }();
+    return slots;
+}
+
 template <int N>
 static hipError_t prepare_n() {
     hipError_t e;
@@ -841,8 +861,11 @@ static hipError_t prepare_n() {
     if ((e = hipFuncSetAttribute((const void*)k_fwd_cols<N>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, colLds)) != hipSuccess)
         return e;
-    return hipFuncSetAttribute((const void*)k_inv_cols<N>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    if ((e = hipFuncSetAttribute((const void*)k_inv_cols<N>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds)) !=
+        hipSuccess)
+        return e;
+    return inv_cols_slots<N>() > 0 ? hipSuccess : hipErrorInvalidValue;
 }
 
 hipError_t prepare_common() {
@@ -915,9 +938,11 @@ hipError_t launch_inv_cols(int FH, const ScaleSet& ss, int FW, int Hc, float2* T
         const size_t lds = IC::LDS;                                                          \
         if (lds > kMaxLds || Hc > N || FW % IC::CB) return hipErrorInvalidValue;            \
         const int nblk = (FW / 2 + IC::CB) / IC::CB;                                         \
-        const int groups = (nblk * ss.n + 7) / 8;                                            \
-        k_inv_cols<N><<<groups * 8 * kNumOri, IC::THREADS, lds, s>>>(ss, FW, Hc, nblk, T,    \
-                                                                      tstride, tw);          \
+        const int units = (nblk * ss.n + 7) / 8 * 8 * 2 * kNumOri;                           \
+        const int grid = std::min(units, inv_cols_slots<N>());                               \
+        if (grid <= 0) return hipErrorInvalidValue;                                          \
+        k_inv_cols<N><<<grid, IC::THREADS, lds, s>>>(ss, FW, Hc, nblk, units, T, tstride,    \
+                                                     tw);                                    \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
