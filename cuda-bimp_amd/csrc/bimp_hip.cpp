@@ -379,7 +379,7 @@ void bimp_default_options(bimp_options* o) {
     o->flags = 0;
     o->precision = BIMP_PRECISION_FP32;
     o->direct_max_lambda = 64.0f;  // every scale of the reference's banks (4 .. 64)
-    o->direct_sigmas = 0.f;
+    o->direct_sigmas = 8.f;  // 8 sigma: inside the 1e-2 map / 99% keypoint bars, 6 sigma is not
 }
 
 int bimp_snap_transform_size(int n) { return n > 0 ? bimph::snap_transform_size(n) : BIMP_E_ARG; }

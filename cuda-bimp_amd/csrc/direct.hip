@@ -19,8 +19,8 @@
 // 4*rho .. 4*rho+3 serves every output row j of the wavefront, with the tap block
 // u0 = 4*rho - j (filter fragments precomputed for every u0).  So each fragment read from
 // LDS feeds kDirRows x kDirGroups MFMAs and each filter fragment kDirGroups of them.
-// The image tile sits in LDS as four copies shifted by 0..3 samples: every lane's 8-sample
-// window is then two aligned 8-byte reads.
+// The image tile sits in LDS as two copies shifted by 0 and 1 sample: every lane's 8-sample
+// window then starts on a 4-byte boundary in one of them and is four 4-byte reads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,12 +73,16 @@ __device__ __forceinline__ int pad_index(int i, int n, int F, int k) {
 
 // One launch covers every scale of a job table (scales of one pyramid level, heaviest first):
 // workgroup b works on the scale whose tile range [tile0, tile0 + tiles) holds b.
-template <class T>
-__global__ void __launch_bounds__(kDirWaves * 64) k_direct(
+// Two wavefronts per SIMD whatever the workgroup size: 4-wave workgroups run two to a CU.
+template <class T, int NW>
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_direct(
         const DirectLevels lv, const DirectJob* __restrict__ jobs, int njobs) {
     typedef typename Mfma<T>::v8 v8;
-    typedef typename Mfma<T>::v4 v4;
-    constexpr int M = kDirRows, NCG = kDirGroups, TY = kDirTY, TX = kDirTX;
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int M = kDirRows, NCG = kDirGroups, TY = kDirTY, TX = kDirTX, NT = NW * 64;
+    // output rows per wavefront, in passes of M rows each
+    constexpr int PASSES = TY / (M * NW);
+    static_assert(PASSES * M * NW == TY, "tile rows split evenly over the wavefronts");
     int js = njobs - 1;
     while (js > 0 && (int)blockIdx.x < jobs[js].tile0) --js;  // uniform
     const DirectJob& job = jobs[js];
@@ -92,17 +96,18 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     extern __shared__ unsigned char smem[];
     T* copies = reinterpret_cast<T*>(smem);
     const int LWP = ds.lwp, csize = ds.cstride;
-    float* vs = reinterpret_cast<float*>(smem + (size_t)4 * csize * sizeof(T));
+    float* vs = reinterpret_cast<float*>(smem + (size_t)2 * csize * sizeof(T));
     const int k = ds.kpad, kW = ds.kW;
     const int TWi = TX + 2 * k;  // columns of the image window the tile needs
     const int x0 = tbx * TX, y0 = tby * TY;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
-    // Stage rows y0-k .. and columns x0-k .. of Pd as four shifted copies (tile sample c at
-    // position c - s of copy s).  Positions past the image window hold zero; they only meet
-    // zero taps.  Wave w takes rows w, w + kDirWaves, ..., lane l columns l, l + 64, l + 128;
-    // the byte loads of kStageRows rows are all issued before any is written, so that their
-    // latencies overlap instead of adding up row by row.
+    // Stage rows y0-k .. and columns x0-k .. of Pd as two copies, the second shifted by one
+    // sample (tile sample c at position c - s of copy s), so that every lane's 8-sample window
+    // starts on a 4-byte boundary in one of them.  Positions past the image window hold zero;
+    // they only meet zero taps.  Wave w takes rows w, w + NW, ..., lane l columns l, l + 64,
+    // ...; the byte loads of kStageRows rows are all issued before any is written, so that
+    // their latencies overlap instead of adding up row by row.
     const bool inside = y0 - k >= 0 && y0 + TY + k <= ds.H && x0 - k >= 0 && x0 + TX + k <= ds.W;
     {
         constexpr int kStageRows = 4, kCols = kDirMaxLwp / 64;
@@ -113,11 +118,11 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
             const int c = lane + 64 * i;
             gxs[i] = c >= TWi ? -1 : (inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k));
         }
-        for (int r0 = wave; r0 < ds.LH; r0 += kStageRows * kDirWaves) {
+        for (int r0 = wave; r0 < ds.LH; r0 += kStageRows * NW) {
             float v[kStageRows][kCols];
 #pragma unroll
             for (int q = 0; q < kStageRows; ++q) {
-                const int r = r0 + q * kDirWaves;
+                const int r = r0 + q * NW;
                 const bool rowin = r < TY + 2 * k;
                 const int gy = !rowin ? 0
                                       : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
@@ -128,7 +133,7 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
             }
 #pragma unroll
             for (int q = 0; q < kStageRows; ++q) {
-                const int r = r0 + q * kDirWaves;
+                const int r = r0 + q * NW;
                 if (r >= ds.LH) break;
                 T* dst = copies + r * LWP;
 #pragma unroll
@@ -136,9 +141,8 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
                     const int c = lane + 64 * i;
                     if (c >= LWP) break;
                     const T tv = T(v[q][i]);
-#pragma unroll
-                    for (int sft = 0; sft < 4; ++sft)
-                        if (c >= sft) dst[sft * csize + c - sft] = tv;
+                    dst[c] = tv;
+                    if (c >= 1) dst[csize + c - 1] = tv;
                 }
             }
         }
@@ -149,7 +153,6 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
     // vs[r][c] (two segments of TY/2 rows per column when the workgroup has a thread for each,
     // else one of TY rows), then horizontal ones, stored back into vs as a TY x TX array.
     {
-        constexpr int NT = kDirWaves * 64;
         const int nseg = 2 * TWi <= NT ? 2 : 1;
         const int rows = TY / nseg;
         for (int t = tid; t < nseg * TWi; t += NT) {
@@ -174,9 +177,11 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
         }
     }
     __syncthreads();
-    float bsum[8];
-    {
-        const int r = tid >> 4, xs = (tid & 15) * 8;
+    constexpr int HP = TY * (TX / 8) / NT;  // horizontal-sum items (8 columns each) per thread
+    float bsum[HP][8];
+#pragma unroll
+    for (int h = 0; h < HP; ++h) {
+        const int t = tid + h * NT, r = t >> 4, xs = (t & 15) * 8;
         const float* vr = vs + r * ds.vsp + xs;
         float s4[4] = {0.f, 0.f, 0.f, 0.f};
         int v = 0;
@@ -186,34 +191,28 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
         }
         for (; v < kW; ++v) s4[0] += vr[v];
         float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        bsum[0] = sum;
+        bsum[h][0] = sum;
 #pragma unroll
         for (int i = 1; i < 8; ++i) {
             sum += vr[i + kW - 1] - vr[i - 1];
-            bsum[i] = sum;
+            bsum[h][i] = sum;
         }
     }
     __syncthreads();
-    {
-        const int r = tid >> 4, xs = (tid & 15) * 8;
+#pragma unroll
+    for (int h = 0; h < HP; ++h) {
+        const int t = tid + h * NT, r = t >> 4, xs = (t & 15) * 8;
         float4* dstb = reinterpret_cast<float4*>(vs + r * TX + xs);
-        dstb[0] = make_float4(bsum[0], bsum[1], bsum[2], bsum[3]);
-        dstb[1] = make_float4(bsum[4], bsum[5], bsum[6], bsum[7]);
+        dstb[0] = make_float4(bsum[h][0], bsum[h][1], bsum[h][2], bsum[h][3]);
+        dstb[1] = make_float4(bsum[h][4], bsum[h][5], bsum[h][6], bsum[h][7]);
     }
+    __syncthreads();  // the box sums are read by the epilogues
 
     // Main loop.  Lane (p, kb): pixel column p of each 16-column group, tap row kb of the
-    // K-step.  The copy index s = (off + p) & 3 is fixed per lane.
+    // K-step.  The copy s = (off + p) & 1 is fixed per lane; its window starts on an even
+    // position, so a fragment is four 4-byte LDS reads.
     const int p = lane & 15, kb = lane >> 4;
-    const int s = (ds.off + p) & 3;
-    const T* lanebase = copies + s * csize + (ds.off + p - s) + (wave * M + ds.off + kb) * LWP;
-    f32x4 acc[M][NCG];
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-#pragma unroll
-        for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // Software-pipelined over the K-steps it = rho * nvb + vb: the image fragments of step
-    // it + 1 (LDS) and the filter fragments of step it + 2 (L2) are in flight while the MFMAs
-    // of step it run.  Fragment offsets advance incrementally (no division per step).
+    const int s = (ds.off + p) & 1;
     const int nvb = ds.nvb, nsteps = ds.nrho * nvb;
     // step cursor: (rho, vb) and the element offsets of its image and filter fragments
     struct Cursor {
@@ -231,83 +230,95 @@ __global__ void __launch_bounds__(kDirWaves * 64) k_direct(
             c.boff += 64;
         }
     };
-    auto load_a = [&](const Cursor& c, v8* a) {
-        const T* rowp = lanebase + c.aoff;
-#pragma unroll
-        for (int g = 0; g < NCG; ++g) {
-            const v4 lo = *reinterpret_cast<const v4*>(rowp + 16 * g);
-            const v4 hi = *reinterpret_cast<const v4*>(rowp + 16 * g + 4);
-            a[g] = v8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-        }
-    };
     auto load_b = [&](const Cursor& c, v8* b) {
         const v8* bro = bbase + c.boff;
 #pragma unroll
         for (int j = 0; j < M; ++j) b[j] = bro[-j * nvb * 64];
     };
-    v8 a[NCG], b[M], b1[M];
-    Cursor ca{0, 0, 0};
-    load_a(ca, a);
-    load_b(ca, b);
-    Cursor cb = ca;
-    if (nsteps > 1) advance(cb);
-    load_b(cb, b1);
-    if (nsteps > 1) advance(ca);  // ca: step it + 1, cb: step it + 2 (both clamped)
-    if (nsteps > 2) advance(cb);
+    const int jf = lane & 15, o = jf >> 1;
+    const float dce = ds.dc[2 * o], dco = ds.dc[2 * o + 1];
 #pragma unroll 1
-    for (int it = 0; it < nsteps; ++it) {
-        v8 an[NCG], bn[M];
-        load_a(ca, an);
-        load_b(cb, bn);
+    for (int pass = 0; pass < PASSES; ++pass) {
+        const int row0 = (wave * PASSES + pass) * M;  // first tile row of this pass
+        const T* lanebase = copies + s * csize + (ds.off + p - s) + (row0 + ds.off + kb) * LWP;
+        auto load_a = [&](const Cursor& c, v8* a) {
+            const unsigned* rowp = reinterpret_cast<const unsigned*>(lanebase + c.aoff);
+#pragma unroll
+            for (int g = 0; g < NCG; ++g) {
+                const u32x4 w = {rowp[8 * g], rowp[8 * g + 1], rowp[8 * g + 2], rowp[8 * g + 3]};
+                a[g] = __builtin_bit_cast(v8, w);
+            }
+        };
+        f32x4 acc[M][NCG];
 #pragma unroll
         for (int j = 0; j < M; ++j)
 #pragma unroll
-            for (int g = 0; g < NCG; ++g) acc[j][g] = Mfma<T>::run(a[g], b[j], acc[j][g]);
+            for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // Software-pipelined over the K-steps it = rho * nvb + vb: the image fragments of step
+        // it + 1 (LDS) and the filter fragments of step it + 2 (L2) are in flight while the
+        // MFMAs of step it run.  Fragment offsets advance incrementally (no division per step).
+        v8 a[NCG], b[M], b1[M];
+        Cursor ca{0, 0, 0};
+        load_a(ca, a);
+        load_b(ca, b);
+        Cursor cb = ca;
+        if (nsteps > 1) advance(cb);
+        load_b(cb, b1);
+        if (nsteps > 1) advance(ca);  // ca: step it + 1, cb: step it + 2 (both clamped)
+        if (nsteps > 2) advance(cb);
+#pragma unroll 1
+        for (int it = 0; it < nsteps; ++it) {
+            v8 an[NCG], bn[M];
+            load_a(ca, an);
+            load_b(cb, bn);
 #pragma unroll
-        for (int g = 0; g < NCG; ++g) a[g] = an[g];
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int g = 0; g < NCG; ++g) acc[j][g] = Mfma<T>::run(a[g], b[j], acc[j][g]);
+#pragma unroll
+            for (int g = 0; g < NCG; ++g) a[g] = an[g];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                b[j] = b1[j];
+                b1[j] = bn[j];
+            }
+            if (it + 2 < nsteps) advance(ca);
+            if (it + 3 < nsteps) advance(cb);
+        }
+
+        // Epilogue: lane holds filter jf = lane%16 for pixels 4*kb .. 4*kb+3 of each group;
+        // the odd filter of orientation jf/2 sits in the neighbouring lane.
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-            b[j] = b1[j];
-            b1[j] = bn[j];
-        }
-        if (it + 2 < nsteps) advance(ca);
-        if (it + 3 < nsteps) advance(cb);
-    }
-    __syncthreads();  // the box sums stored before the main loop are read below
-
-    // Epilogue: lane holds filter jf = lane%16 for pixels 4*kb .. 4*kb+3 of each group; the
-    // odd filter of orientation jf/2 sits in the neighbouring lane.
-    const int jf = lane & 15, o = jf >> 1;
-    const float dce = ds.dc[2 * o], dco = ds.dc[2 * o + 1];
+            const int yl = row0 + j, y = y0 + yl;
 #pragma unroll
-    for (int j = 0; j < M; ++j) {
-        const int yl = wave * M + j, y = y0 + yl;
+            for (int g = 0; g < NCG; ++g) {
+                const int xl = 16 * g + 4 * kb, x = x0 + xl;
+                // the lane pair (even filter, odd filter) splits the four pixels: the even
+                // lane finishes pixels 0-1, the odd lane pixels 2-3, after swapping the other
+                // halves
+                const int tb2 = (jf & 1) ? 2 : 0;
+                const float2 bx = *reinterpret_cast<const float2*>(vs + yl * TX + xl + tb2);
+                const float bxs[2] = {bx.x, bx.y};
+                float c2[2];
 #pragma unroll
-        for (int g = 0; g < NCG; ++g) {
-            const int xl = 16 * g + 4 * kb, x = x0 + xl;
-            // the lane pair (even filter, odd filter) splits the four pixels: the even lane
-            // finishes pixels 0-1, the odd lane pixels 2-3, after swapping the other halves
-            const int tb = (jf & 1) ? 2 : 0;
-            const float2 bx = *reinterpret_cast<const float2*>(vs + yl * TX + xl + tb);
-            const float bxs[2] = {bx.x, bx.y};
-            float c2[2];
+                for (int q = 0; q < 2; ++q) {
+                    const float mine = (jf & 1) ? acc[j][g][2 + q] : acc[j][g][q];
+                    const float send = (jf & 1) ? acc[j][g][q] : acc[j][g][2 + q];
+                    const float other = __shfl_xor(send, 1, 64);
+                    const float e = ds.fscale * ((jf & 1) ? other : mine) - dce * bxs[q];
+                    const float od = ds.fscale * ((jf & 1) ? mine : other) - dco * bxs[q];
+                    c2[q] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
+                }
+                if (y < ds.Hc) {
+                    float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x + tb2;
+                    if (x + tb2 + 1 < ds.Wc) {
+                        *reinterpret_cast<float2*>(out) = make_float2(c2[0], c2[1]);
+                    } else {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const float mine = (jf & 1) ? acc[j][g][2 + q] : acc[j][g][q];
-                const float send = (jf & 1) ? acc[j][g][q] : acc[j][g][2 + q];
-                const float other = __shfl_xor(send, 1, 64);
-                const float e = ds.fscale * ((jf & 1) ? other : mine) - dce * bxs[q];
-                const float od = ds.fscale * ((jf & 1) ? mine : other) - dco * bxs[q];
-                c2[q] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
-            }
-            if (y < ds.Hc) {
-                float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x + tb;
-                if (x + tb + 1 < ds.Wc) {
-                    *reinterpret_cast<float2*>(out) = make_float2(c2[0], c2[1]);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 2; ++q)
-                        if (x + tb + q < ds.Wc) out[q] = c2[q];
+                        for (int q = 0; q < 2; ++q)
+                            if (x + tb2 + q < ds.Wc) out[q] = c2[q];
+                    }
                 }
             }
         }
@@ -325,22 +336,33 @@ size_t direct_geometry(DirectScale& d) {
     const int k = d.kpad;
     d.LH = std::max(kDirTY + 2 * k, (kDirTY - kDirRows) + d.off + 4 * d.nrho);
     const int twi = kDirTX + 2 * k;
-    const int tws = std::max(twi, d.off + kDirTX + extp - 1);
-    d.lwp = (tws + 4 + 3) / 4 * 4;
+    // widest position read (off + TX + extp - 2 in copy 0) plus one sample of slack, since
+    // the shifted copy leaves its last position unwritten
+    const int tws = std::max(twi, d.off + kDirTX + extp - 1) + 2;
+    // row pitch 16 (mod 32) elements: the 4 tap rows of a read start 8 x (odd) banks apart
+    d.lwp = tws + ((16 - tws % 32) + 32) % 32;
     d.vsp = twi;
-    // copy stride: 16 (mod 64) elements puts the four shifted copies 8 LDS banks apart, so the
-    // 16 lanes of one 8-byte read (4 copies x 4 windows) hit 32 distinct banks
+    // copy stride 64 (mod 128) elements: the two copies sit 32 banks apart, so the 64 lanes
+    // of a 4-byte read (4 tap rows x 2 copies x 8 consecutive words) hit 64 distinct banks
     const int cs = d.LH * d.lwp;
-    d.cstride = cs + ((16 - cs % 64) + 64) % 64;
-    return (size_t)4 * d.cstride * 2 + (size_t)kDirTY * d.vsp * sizeof(float);
+    d.cstride = cs + ((64 - cs % 128) + 128) % 128;
+    return (size_t)2 * d.cstride * 2 + (size_t)kDirTY * d.vsp * sizeof(float);
 }
 
+// Workgroup size of a launch: 4 wavefronts (two passes of 4 rows each) when two workgroups
+// fit one CU's LDS, so that one's staging and epilogue overlap the other's MFMA loop;
+// otherwise 8 (one pass).
+static int direct_waves(size_t lds) { return 2 * lds + 2048 <= 160 * 1024 ? 4 : 8; }
+
 hipError_t prepare_direct() {
-    hipError_t e = hipFuncSetAttribute((const void*)k_direct<__bf16>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)k_direct<_Float16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    const void* fns[] = {(const void*)k_direct<__bf16, 4>, (const void*)k_direct<__bf16, 8>,
+                         (const void*)k_direct<_Float16, 4>, (const void*)k_direct<_Float16, 8>};
+    for (const void* f : fns) {
+        const hipError_t e =
+            hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const* C,
@@ -379,10 +401,15 @@ size_t direct_jobs(const DirectScale* ds, const void* const* frag, float* const*
 hipError_t launch_direct(const DirectLevels& lv, const DirectJob* d_jobs, int njobs,
                          int total_tiles, size_t lds, bool fp16, hipStream_t s) {
     if (lds > kMaxLds || njobs < 1 || total_tiles < 1) return hipErrorInvalidValue;
-    if (fp16)
-        k_direct<_Float16><<<total_tiles, kDirWaves * 64, lds, s>>>(lv, d_jobs, njobs);
+    const bool w4 = direct_waves(lds) == 4;
+    if (fp16 && w4)
+        k_direct<_Float16, 4><<<total_tiles, 4 * 64, lds, s>>>(lv, d_jobs, njobs);
+    else if (fp16)
+        k_direct<_Float16, 8><<<total_tiles, 8 * 64, lds, s>>>(lv, d_jobs, njobs);
+    else if (w4)
+        k_direct<__bf16, 4><<<total_tiles, 4 * 64, lds, s>>>(lv, d_jobs, njobs);
     else
-        k_direct<__bf16><<<total_tiles, kDirWaves * 64, lds, s>>>(lv, d_jobs, njobs);
+        k_direct<__bf16, 8><<<total_tiles, 8 * 64, lds, s>>>(lv, d_jobs, njobs);
     return hipGetLastError();
 }
 

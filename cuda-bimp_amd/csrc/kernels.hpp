@@ -42,20 +42,19 @@ struct NmsScale {
 constexpr size_t kMaxLds = 160 * 1024 - 1024;
 
 // One scale of the precision form (direct MFMA convolution, see direct.hip).
-// Workgroup tile: kDirWaves wavefronts x kDirRows output rows, kDirGroups x 16 columns.
-constexpr int kDirRows = 4, kDirGroups = 8, kDirWaves = 8;
-constexpr int kDirTY = kDirWaves * kDirRows, kDirTX = kDirGroups * 16;
+// Workgroup tile: kDirTY output rows (passes of kDirRows rows per wavefront) x kDirGroups x 16
+// columns.
+constexpr int kDirRows = 4, kDirGroups = 8;
+constexpr int kDirTY = 32, kDirTX = kDirGroups * 16;
 // widest staged image row (elements): tile columns plus the kernel's 55-tap support, rounded
-constexpr int kDirMaxLwp = 192;
-// the horizontal box-sum pass gives each thread 8 columns of one output row
-static_assert(kDirWaves * 64 == kDirTY * (kDirTX / 8), "one thread per 8 tile columns");
+constexpr int kDirMaxLwp = 256;
 struct DirectScale {
     int H, W, FH, FW;        // data size, transform size (the periodic domain)
     int kpad, kW;            // kernel centre kX (clamp padding, box half-width) and width
     int Hc, Wc, ldc;         // complex-cell map region and pitch
     int off, ext, nvb, nrho; // tap window offset kX - R, extent, 8-column blocks, 4-row blocks
     int LH, lwp, vsp;        // staged rows, LDS row pitch (elements), box-sum row pitch (floats)
-    int cstride;             // element stride between the four shifted copies
+    int cstride;             // element stride between the two shifted copies
     float fscale, iscale;    // filter scale, input scale (255/65536)
     float dc[2 * kNumOri];   // box coefficients, filter 2o even, 2o+1 odd
 };

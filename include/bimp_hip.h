@@ -72,9 +72,9 @@ typedef struct bimp_options {
     float direct_max_lambda; /* low-precision forms: scales whose requested wavelength is
                                 at most this (full-resolution pixels) run the direct form */
     float direct_sigmas;  /* low-precision forms: 0 convolves with the full kW x kW kernel;
-                             s > 0 truncates the taps to an odd extent of ceil(s*sigma).
-                             Either way an exact integer box sum restores every filter's
-                             response to a constant image */
+                             s > 0 truncates the taps to an odd extent of ceil(s*sigma)
+                             (default 8).  Either way an exact integer box sum restores every
+                             filter's response to a constant image */
 } bimp_options;
 
 #define BIMP_FLAG_PROFILE 1  /* time every kernel with HIP events (see bimp_get_profile) */
