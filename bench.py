@@ -10,7 +10,8 @@ The same line carries the precision form (BF16 / FP16 direct convolution on the 
 for every scale): its frames/s, speed-up over the plain
 path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle,
 the batching form (64 frames of five sizes as one tuned hipGraph launch): aggregate
-megapixels/s and per-frame latency, and the differentiation form (16 FP32 512x512 images
+megapixels/s and per-frame latency, the headless video loop (host frames in, keypoints out,
+frames/s), and the differentiation form (16 FP32 512x512 images
 through the libtorch autograd Function): backward time per image and gradient checks.
 """
 import argparse
@@ -191,6 +192,26 @@ def batch_form(B, stream, frames=64, seed=3, iters=3):
     }
     bt.close()
     return out
+
+
+def video_form(B, cs, frames=200, seq=16):
+    """Headless video loop (videoKeypoints, example_bimp.cpp:116-196, without capture, resize
+    or rendering): each frame is a host uint8 array that is uploaded, filtered and reduced to
+    keypoints, which come back to the host before the next frame, as the reference's loop
+    does.  The sequence is 16 synthetic 1920x1080 frames (300 shapes, seeds 3000 + i), cycled."""
+    seqf = [B.synth_frame(W, H, NSHAPES, 3000 + i) for i in range(seq)]
+    for f in seqf[:2]:
+        cs.detect(f)
+    t0 = time.perf_counter()
+    total = 0
+    for i in range(frames):
+        _, n = cs.detect(seqf[i % seq])
+        total += n
+    dt = time.perf_counter() - t0
+    return {"frames": frames, "sequence": f"{seq} synthetic frames cycled",
+            "fps_host_to_host": round(frames / dt, 1), "ms_per_frame": round(dt / frames * 1e3, 4),
+            "keypoints_per_frame": round(total / frames, 1),
+            "note": "pageable host frame in, keypoint records out to host, one frame at a time"}
 
 
 def diff_form(B, nimg=16, size=512, check_fp64=True):
@@ -441,6 +462,10 @@ def main():
             best = max(ok, key=lambda n: forms[n]["fps"])
             out["precision_form_best"] = {"form": best, "fps": forms[best]["fps"],
                                           "speedup_vs_plain": forms[best]["speedup_vs_plain"]}
+    if args.form == "plain":
+        out["video_form"] = video_form(B, cs)
+        print(f"video form: {out['video_form']['fps_host_to_host']} frames/s host to host",
+              file=sys.stderr, flush=True)
     if not args.no_batch and args.form == "plain":
         out["batch_form"] = batch_form_child()
         print(f"batch form: {out['batch_form'].get('mpix_per_s')} MPix/s", file=sys.stderr,
