@@ -110,6 +110,8 @@ void free_all(bimp_ctx* c) {
     (void)hipFree(c->blockCounts);
     (void)hipFree(c->nmsTable);
     (void)hipFree(c->kpBuf);
+    if (c->hostIn) (void)hipHostFree(c->hostIn);
+    if (c->hostInDone) (void)hipEventDestroy(c->hostInDone);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -155,7 +157,22 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
     size_t pitch0 = pitch;
     if (!on_device) {
         LaunchScope ls(c, s, F_COPY_IN, 2.0 * W0 * H0);
-        HIPCHK(c, hipMemcpy2DAsync(c->level[0], W0, img, pitch, W0, H0, hipMemcpyHostToDevice, s));
+        const size_t bytes = (size_t)W0 * H0;
+        if (pitch == (size_t)W0) {
+            HIPCHK(c, hipMemcpyAsync(c->level[0], img, bytes, hipMemcpyHostToDevice, s));
+        } else {
+            // pitched host frame: rows into pinned staging, then one contiguous upload
+            if (!c->hostIn) {
+                HIPCHK(c, hipHostMalloc((void**)&c->hostIn, bytes, hipHostMallocDefault));
+                HIPCHK(c, hipEventCreateWithFlags(&c->hostInDone, hipEventDisableTiming));
+            } else {
+                HIPCHK(c, hipEventSynchronize(c->hostInDone));
+            }
+            for (int y = 0; y < H0; ++y)
+                std::memcpy(c->hostIn + (size_t)y * W0, img + (size_t)y * pitch, (size_t)W0);
+            HIPCHK(c, hipMemcpyAsync(c->level[0], c->hostIn, bytes, hipMemcpyHostToDevice, s));
+            HIPCHK(c, hipEventRecord(c->hostInDone, s));
+        }
         src0 = c->level[0];
         pitch0 = W0;
     }

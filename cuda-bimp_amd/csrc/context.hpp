@@ -87,6 +87,10 @@ struct bimp_ctx {
     bimpk::NmsScale* nmsTable = nullptr;
     void* kpBuf = nullptr;
     size_t kpBufBytes = 0;
+    // pinned staging of pitched host frames (a 2D copy from pageable memory is slow), and the
+    // event after its last upload, waited on before the staging is rewritten
+    uint8_t* hostIn = nullptr;
+    hipEvent_t hostInDone = nullptr;
     bool maxValid = false;
     std::string err;
     // profiling

@@ -81,14 +81,14 @@ __global__ void __launch_bounds__(64) k_fwd_rows(const uint8_t* __restrict__ img
     float* my = lds + (size_t)g * 2 * FW;
     const uint8_t* rowA = img + (size_t)(haveA ? ya : 0) * pitch;
     const uint8_t* rowB = img + (size_t)(haveB ? yb : 0) * pitch;
+    // unconditional loads (rows clamped above) so that all of them are in flight at once
+    const float sa = haveA ? scale : 0.f, sb = haveB ? scale : 0.f;
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) {
         const int x = in_index<FW>(e, lane);
         const int cx = x < W ? x : (x < W + kX ? W - 1 : 0);
-        const float a = haveA ? (float)rowA[cx] * scale : 0.f;
-        const float b = haveB ? (float)rowB[cx] * scale : 0.f;
-        v[e] = make_float2(a, b);
+        v[e] = make_float2((float)rowA[cx] * sa, (float)rowB[cx] * sb);
     }
     fft<FW, false>(v, my, lane, tw);
     float2* z = reinterpret_cast<float2*>(my);
@@ -372,11 +372,15 @@ __device__ __forceinline__ void cells_fetch(const MapView& mv, int H, int W, int
             if (INTERIOR) {
                 v = *reinterpret_cast<const float4*>(mv.base + ((size_t)o * mv.Hc + gy) * mv.ldc + gx);
             } else {
+                // unconditional loads (outside positions read element (0, 0) and are zeroed
+                // after), so that a border tile's loads are all in flight at once
                 float e[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    e[j] = (gy >= 0 && gy <= H && gx + j >= 0 && gx + j <= W) ? mv.at(o, gy, gx + j)
-                                                                                : 0.f;
+                for (int j = 0; j < 4; ++j) {
+                    const bool ok = gy >= 0 && gy <= H && gx + j >= 0 && gx + j <= W;
+                    const float t = mv.at(o, ok ? gy : 0, ok ? gx + j : 0);
+                    e[j] = ok ? t : 0.f;
+                }
                 v = make_float4(e[0], e[1], e[2], e[3]);
             }
         }

@@ -53,3 +53,25 @@ def test_cpu_keypoint_mode_matches_oracle(B, w, h):
     want = np.concatenate(want)
     assert n == m and n > 0
     assert np.array_equal(bits(got), bits(want))
+
+
+def test_pitched_host_frame_equals_contiguous(B):
+    """A host frame with a row pitch wider than its width (staged through pinned memory)
+    gives the keypoints of the same frame passed contiguously."""
+    import ctypes as C
+
+    import oracle_lib as O
+    w, h = 200, 150
+    img = O.synth_frame(w, h, 12, 7)
+    cs = B.CudaStuff(w, h, O.BASE_BANK)
+    want, wn = cs.detect(img)
+    padded = np.full((h, w + 56), 255, np.uint8)
+    padded[:, :w] = img
+    for _ in range(2):  # the second call reuses the staging buffer
+        out = np.zeros(B.KP_MAX_POINTS, B.KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        st = B.lib.bimp_detect_u8(cs.ctx, B._ptr(padded), w + 56, 0, B._ptr(out), B.KP_MAX_POINTS,
+                                  0, C.byref(n), None)
+        assert st >= 0
+        assert n.value == wn
+        assert np.array_equal(out[:wn].view(np.uint32), want.view(np.uint32))
