@@ -332,15 +332,16 @@ __device__ __forceinline__ Contrib cell_terms(const float* __restrict__ m,
                                               const float* __restrict__ m2,
                                               const CellTileOffsets& q, int o) {
 #pragma clang fp contract(off)
+    // Products by 0.5 and 16 are exact, so fma(-0.5, a, c) and fma(-16, m, c) round once
+    // exactly where the reference's separate multiply and subtract do: same bits, fewer ops.
     Contrib c;
     const float centre = m[0];
     c.s = fmaxf(m[q.es1[o]] - m[q.es2[o]], 0.f);
-    c.d = fmaxf((float)((double)centre - 0.5 * (double)m[q.ed1[o]] -
-                        0.5 * (double)m[q.ed2[o]]),
-                0.f);
+    const double d1 = __builtin_fma(-0.5, (double)m[q.ed1[o]], (double)centre);
+    c.d = fmaxf((float)__builtin_fma(-0.5, (double)m[q.ed2[o]], d1), 0.f);
     c.t = fmaxf(m[q.tn[o]] - centre, 0.f);
-    float v = fmaxf(centre - 16.f * m2[q.rp[o]], 0.f);
-    v += centre - 16.f * m2[q.rm[o]];
+    float v = fmaxf(__builtin_fmaf(-16.f, m2[q.rp[o]], centre), 0.f);
+    v += __builtin_fmaf(-16.f, m2[q.rm[o]], centre);
     c.r = fmaxf(v, 0.f);
     return c;
 }
@@ -372,15 +373,11 @@ __device__ __forceinline__ void cells_fetch(const MapView& mv, int H, int W, int
             if (INTERIOR) {
                 v = *reinterpret_cast<const float4*>(mv.base + ((size_t)o * mv.Hc + gy) * mv.ldc + gx);
             } else {
-                // unconditional loads (outside positions read element (0, 0) and are zeroed
-                // after), so that a border tile's loads are all in flight at once
                 float e[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const bool ok = gy >= 0 && gy <= H && gx + j >= 0 && gx + j <= W;
-                    const float t = mv.at(o, ok ? gy : 0, ok ? gx + j : 0);
-                    e[j] = ok ? t : 0.f;
-                }
+                for (int j = 0; j < 4; ++j)
+                    e[j] = (gy >= 0 && gy <= H && gx + j >= 0 && gx + j <= W) ? mv.at(o, gy, gx + j)
+                                                                                : 0.f;
                 v = make_float4(e[0], e[1], e[2], e[3]);
             }
         }
