@@ -115,11 +115,12 @@ def oracle_reference(img, direct_scales):
 
 
 def precision_form(B, name, prec, nsig, plain_ms, ref_kps, ref_maps, d_img, d_out, cap,
-                   steps, warmup, stream, img, profile_frames):
+                   steps, warmup, stream, img, profile_frames, max_lambda=64.0):
     from kpmatch import agreement, rel_err
     opt = B.default_options()
     opt.precision = prec
     opt.direct_sigmas = nsig
+    opt.direct_max_lambda = max_lambda
     cs = B.CudaStuff(W, H, bank(), opt)
     ms = time_form(B, cs, d_img, d_out, cap, steps, warmup, stream)
     # every keypoint of the frame (the reference's 8000 cap would truncate the comparison)
@@ -144,6 +145,7 @@ def precision_form(B, name, prec, nsig, plain_ms, ref_kps, ref_maps, d_img, d_ou
         "fps": round(1000.0 / ms, 2), "ms_per_step": round(ms, 4),
         "speedup_vs_plain": round(plain_ms / ms, 3),
         "extent": "full kW" if nsig <= 0 else f"{nsig:g} sigma",
+        "direct_max_lambda": max_lambda,
         "direct_scales": ["%.4f" % si.lambda2 for si in cs.ks if si.precision != B.PRECISION_FP32],
         "tap_extents": [si.tap_extent for si in cs.ks if si.precision != B.PRECISION_FP32],
         "keypoints": int(count),
@@ -444,13 +446,18 @@ def main():
               flush=True)
         out["keypoint_agreement_vs_fp64_oracle"] = round(agreement(kps, ref_kps), 4)
         forms = {}
-        for name, prec, nsig in (("bf16", B.PRECISION_BF16, 0.0), ("fp16", B.PRECISION_FP16, 0.0),
-                                 ("bf16_8sigma", B.PRECISION_BF16, 8.0),
-                                 ("fp16_8sigma", B.PRECISION_FP16, 8.0),
-                                 ("bf16_6sigma", B.PRECISION_BF16, 6.0)):
+        # every scale direct (the library's default) at full kW, 8 sigma and 6 sigma, and the
+        # split the workload names: lambda <= 8 direct at 6 sigma, the rest FFT-FP32
+        for name, prec, nsig, maxl in (("bf16", B.PRECISION_BF16, 0.0, 64.0),
+                                       ("fp16", B.PRECISION_FP16, 0.0, 64.0),
+                                       ("bf16_8sigma", B.PRECISION_BF16, 8.0, 64.0),
+                                       ("fp16_8sigma", B.PRECISION_FP16, 8.0, 64.0),
+                                       ("bf16_6sigma", B.PRECISION_BF16, 6.0, 64.0),
+                                       ("bf16_6sigma_lambda8", B.PRECISION_BF16, 6.0, 8.0),
+                                       ("fp16_6sigma_lambda8", B.PRECISION_FP16, 6.0, 8.0)):
             forms[name] = precision_form(B, name, prec, nsig, ms_per_step, ref_kps, ref_maps,
                                          d_img, d_out, cap, args.steps, args.warmup, stream, img,
-                                         args.profile_frames)
+                                         args.profile_frames, maxl)
             print(f"precision form {name}: {forms[name]['fps']} frames/s", file=sys.stderr,
                   flush=True)
         out["precision_form"] = forms
