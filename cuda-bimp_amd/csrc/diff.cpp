@@ -20,8 +20,8 @@
 namespace bimpk {
 hipError_t diff_prepare(int n);
 hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, hipStream_t s);
-hipError_t diff_transpose(const float2* src, float2* dst, int rows, int cols, int batch,
-                          hipStream_t s);
+hipError_t diff_cols(float2* data, int n, int w, int batch, bool inv, const float2* tw,
+                     hipStream_t s);
 hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
                          hipStream_t s);
 hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int sw, int sh,
@@ -74,7 +74,7 @@ struct bimp_diff {
     std::vector<float*> lev, glev;
     std::map<int, float2*> tw;
     size_t Pmax = 0;
-    float2 *Dhat = nullptr, *Z = nullptr, *T = nullptr, *acc = nullptr;
+    float2 *Dhat = nullptr, *Z = nullptr, *acc = nullptr;
     float *C = nullptr, *Wm = nullptr, *gC = nullptr, *S = nullptr, *taps = nullptr;
     unsigned* maxslot = nullptr;
     float* partials = nullptr;
@@ -99,9 +99,7 @@ int dfail(bimp_diff* d, int code, const std::string& m) {
 // In-place 2D complex transform of `nb` FH x FW maps (scratch of the same size).
 int fft2(bimp_diff* d, float2* a, int nb, int FH, int FW, bool inv, hipStream_t s) {
     DCHK(d, bimpk::diff_rows(a, nb * FH, FW, inv, d->tw[FW], s));
-    DCHK(d, bimpk::diff_transpose(a, d->T, FH, FW, nb, s));
-    DCHK(d, bimpk::diff_rows(d->T, nb * FW, FH, inv, d->tw[FH], s));
-    DCHK(d, bimpk::diff_transpose(d->T, a, FW, FH, nb, s));
+    DCHK(d, bimpk::diff_cols(a, FH, FW, nb, inv, d->tw[FH], s));
     return BIMP_OK;
 }
 
@@ -239,7 +237,6 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
     if (hipMalloc(&d->Dhat, P * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->acc, P * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->Z, kNumOri * P * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&d->T, kNumOri * P * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->C, kNumOri * P * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->gC, kNumOri * P * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->Wm, 4 * kNumOri * P * sizeof(float)) != hipSuccess ||
@@ -278,7 +275,7 @@ int bimp_diff_destroy(bimp_diff* d) {
     for (auto* p : d->lev) (void)hipFree(p);
     for (auto* p : d->glev) (void)hipFree(p);
     for (auto& kv : d->tw) (void)hipFree(kv.second);
-    for (void* p : {(void*)d->Dhat, (void*)d->acc, (void*)d->Z, (void*)d->T, (void*)d->C,
+    for (void* p : {(void*)d->Dhat, (void*)d->acc, (void*)d->Z, (void*)d->C,
                     (void*)d->gC, (void*)d->Wm, (void*)d->S, (void*)d->taps, (void*)d->maxslot,
                     (void*)d->partials})
         (void)hipFree(p);

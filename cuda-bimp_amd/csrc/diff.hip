@@ -11,8 +11,8 @@
 //   gPd  = Re IFFT(sum_o conj(Qhat_o) * Ghat_o) / N,
 //   Y_o  = IFFT(Ghat_o * conj(FFT(Pd))) / N = dL/dKe_o + i dL/dKo_o at the tap positions,
 //   then gL = s * pad^T(gPd) and the float-pyramid adjoint down to the image.
-// All transforms are full complex 2D FFTs built from batched row transforms and tiled
-// transposes.
+// All transforms are full complex 2D FFTs: batched row transforms, then column transforms
+// staged through LDS.
 #include <hip/hip_runtime.h>
 
 #include "fft_device.hpp"
@@ -47,21 +47,45 @@ __global__ void __launch_bounds__(kRowThreads) k_c2c_rows(float2* __restrict__ d
     for (int e = 0; e < C::E; ++e) p[out_index<N>(e, lane)] = v[e];
 }
 
-// dst[b][c][r] = src[b][r][c] for rows x cols matrices, 32 x 32 tiles through LDS.
-__global__ void __launch_bounds__(256) k_transpose(const float2* __restrict__ src,
-                                                   float2* __restrict__ dst, int rows, int cols) {
-    __shared__ float2 t[32][33];
-    const size_t boff = (size_t)blockIdx.z * rows * cols;
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int k = ty; k < 32; k += 8) {
-        const int r = r0 + k, c = c0 + tx;
-        if (r < rows && c < cols) t[k][tx] = src[boff + (size_t)r * cols + c];
+// In-place complex transforms of the columns of `nb` N x W maps ([b][r][c], r < N): a
+// workgroup stages CB adjacent columns through LDS (each row contributes CB contiguous
+// values), one group of lanes transforms each column, and the results leave the same way.
+// It replaces transpose + row transforms + transpose and moves each value once each way.
+template <int N>
+struct CColCfg {
+    static constexpr int TPV = FFTCfg<N>::TPV;
+    static constexpr int CB = TPV == 64 ? (N <= 2048 ? 8 : 4) : (512 / TPV > 16 ? 16 : 512 / TPV);
+    static constexpr int THREADS = CB * TPV;
+    static constexpr int S = N + 2;  // column stride (complex): 2 (mod 16) spreads the staging
+};
+
+template <int N, bool INV>
+__global__ void __launch_bounds__(CColCfg<N>::THREADS) k_c2c_cols(float2* __restrict__ data,
+                                                                   int W,
+                                                                   const float2* __restrict__ tw) {
+    using C = FFTCfg<N>;
+    using CC = CColCfg<N>;
+    extern __shared__ float2 lds2[];
+    float2* map = data + (size_t)blockIdx.y * N * W;
+    const int c0 = blockIdx.x * CC::CB;
+    for (int t = threadIdx.x; t < N * CC::CB; t += CC::THREADS) {
+        const int b = t % CC::CB, r = t / CC::CB;
+        lds2[b * CC::S + r] = c0 + b < W ? map[(size_t)r * W + c0 + b] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    for (int k = ty; k < 32; k += 8) {
-        const int c = c0 + k, r = r0 + tx;
-        if (r < rows && c < cols) dst[boff + (size_t)c * rows + r] = t[tx][k];
+    const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
+    float2* my = lds2 + (size_t)g * CC::S;
+    float2 v[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) v[e] = my[in_index<N>(e, lane)];
+    bimpdev::stage_sync();
+    bimpdev::fft<N, INV>(v, reinterpret_cast<float*>(my), lane, tw);
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) my[out_index<N>(e, lane)] = v[e];
+    __syncthreads();
+    for (int t = threadIdx.x; t < N * CC::CB; t += CC::THREADS) {
+        const int b = t % CC::CB, r = t / CC::CB;
+        if (c0 + b < W) map[(size_t)r * W + c0 + b] = lds2[b * CC::S + r];
     }
 }
 
@@ -381,11 +405,19 @@ hipError_t diff_prepare(int n) {
 #define X(N)                                                                                 \
     if (n == N) {                                                                            \
         const int lds = (kRowThreads / FFTCfg<N>::TPV) * bimpdev::fft_lds_floats<N>() * (int)sizeof(float); \
+        const int clds = CColCfg<N>::CB * CColCfg<N>::S * (int)sizeof(float2);               \
         hipError_t e = hipFuncSetAttribute((const void*)k_c2c_rows<N, false>,                \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-        if (e != hipSuccess) return e;                                                       \
-        return hipFuncSetAttribute((const void*)k_c2c_rows<N, true>,                         \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);         \
+        if (e == hipSuccess)                                                                 \
+            e = hipFuncSetAttribute((const void*)k_c2c_rows<N, true>,                        \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);        \
+        if (e == hipSuccess)                                                                 \
+            e = hipFuncSetAttribute((const void*)k_c2c_cols<N, false>,                       \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, clds);       \
+        if (e == hipSuccess)                                                                 \
+            e = hipFuncSetAttribute((const void*)k_c2c_cols<N, true>,                        \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, clds);       \
+        return e;                                                                            \
     }
     BIMP_FOR_SIZES(X)
 #undef X
@@ -409,11 +441,22 @@ hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, 
     return hipErrorInvalidValue;
 }
 
-hipError_t diff_transpose(const float2* src, float2* dst, int rows, int cols, int batch,
-                          hipStream_t s) {
-    dim3 grid((cols + 31) / 32, (rows + 31) / 32, batch);
-    k_transpose<<<grid, 256, 0, s>>>(src, dst, rows, cols);
-    return hipGetLastError();
+hipError_t diff_cols(float2* data, int n, int w, int batch, bool inv, const float2* tw,
+                     hipStream_t s) {
+#define X(N)                                                                                 \
+    if (n == N) {                                                                            \
+        using CC = CColCfg<N>;                                                               \
+        const int lds = CC::CB * CC::S * (int)sizeof(float2);                                \
+        dim3 grid((w + CC::CB - 1) / CC::CB, batch);                                         \
+        if (inv)                                                                             \
+            k_c2c_cols<N, true><<<grid, CC::THREADS, lds, s>>>(data, w, tw);                 \
+        else                                                                                 \
+            k_c2c_cols<N, false><<<grid, CC::THREADS, lds, s>>>(data, w, tw);                \
+        return hipGetLastError();                                                            \
+    }
+    BIMP_FOR_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
