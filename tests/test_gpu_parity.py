@@ -173,6 +173,24 @@ def test_cfg0_640x480(B):
     assert a >= 0.99
 
 
+def test_cfg1_1920x1080(B):
+    """The headline workload at its full size: 1920x1080, 300 shapes, seed 1, full bank; every
+    keypoint (no 8000 cap) against the FP64 oracle."""
+    import os
+    O.lib().bo_set_threads(min(16, os.cpu_count() or 1))
+    w, h = 1920, 1080
+    cs, orc = _pair(B, w, h, O.BASE_BANK)
+    img = B.synth_frame(w, h, 300, 1)
+    kps, n, Ds, Ss = orc.detect(img, cap=1 << 16, want_maps=True)
+    got, gn = cs.detect(img, cap=1 << 16)
+    errs = [rel_err(cs.get_map(i, B.MAP_DOUBLE), Ds[i]) for i in range(8)]
+    print("D rel errs", ["%.1e" % e for e in errs], flush=True)
+    assert max(errs) < MAP_TOL
+    a = agreement(got, kps)
+    print(f"keypoints gpu {gn} oracle {n} agreement {a:.4f}", flush=True)
+    assert gn > 8000 and a >= 0.99
+
+
 def test_repeat_frames_identical(B):
     w, h = 320, 240
     cs = B.CudaStuff(w, h, O.BASE_BANK)
