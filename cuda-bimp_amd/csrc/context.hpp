@@ -1,6 +1,6 @@
 // context.hpp -- the per-frame-size context behind the C ABI (the reference's CudaStuff,
-// bimp/src/cuda_bimp.cu:222-460) and the frame driver shared by the single-frame entry points
-// and the batching form.
+// bimp/src/cuda_bimp.cu:222-460) and the per-frame launch sequence shared by the single-frame
+// entry points and the batching form.
 #pragma once
 #include <hip/hip_runtime.h>
 
