@@ -86,7 +86,8 @@ def time_form(B, cs, d_img, d_out, cap, steps, warmup, stream):
 
 
 def kernel_profile(B, cs, d_img, d_out, cap, frames, stream):
-    cs.set_flags(B.FLAG_PROFILE)
+    # one stream while profiling, so that each launch's duration is its own
+    cs.set_flags(B.FLAG_PROFILE | B.FLAG_SINGLE_STREAM)
     cs.reset_profile()
     cs.time_detect(d_img, W, d_out, cap, frames, stream)
     prof = cs.profile()
@@ -346,6 +347,10 @@ def main():
     ap.add_argument("--form", choices=["plain", "bf16", "fp16", "bf16_8sigma", "fp16_8sigma"],
                     default="plain",
                     help="path timed for the main line (plain = FFT-FP32, the headline)")
+    ap.add_argument("--single-stream", action="store_true",
+                    help="run the main line's frames on one stream (no overlap of the pyramid "
+                         "levels' transforms); profiler runs use it so that kernel durations "
+                         "match the per-kernel profile")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, launch the library's plain copy kernel once over MIB "
                          "MiB (the reference point for HBM byte counters)")
@@ -369,6 +374,8 @@ def main():
     if args.form != "plain":
         opt.precision = B.PRECISION_BF16 if args.form.startswith("bf16") else B.PRECISION_FP16
         opt.direct_sigmas = 8.0 if args.form.endswith("_8sigma") else 0.0
+    base_flags = B.FLAG_SINGLE_STREAM if args.single_stream else 0
+    opt.flags = base_flags
     cs = B.CudaStuff(W, H, bank(), opt)
     stream = B.Stream()
     d_img = B.DeviceBuffer(W * H)
@@ -389,12 +396,13 @@ def main():
     # every keypoint of the frame (the reference's 8000 cap would truncate the comparison)
     kps, count = cs.detect(img, cap=1 << 16)
 
-    # per-kernel profile (HIP events around every launch), separate from the timed region
-    cs.set_flags(B.FLAG_PROFILE)
+    # per-kernel profile (HIP events around every launch), separate from the timed region and
+    # on one stream, so that each launch's duration is its own
+    cs.set_flags(B.FLAG_PROFILE | B.FLAG_SINGLE_STREAM)
     cs.reset_profile()
     cs.time_detect(d_img, W, d_out, cap, args.profile_frames, stream)
     prof = cs.profile()
-    cs.set_flags(0)
+    cs.set_flags(base_flags)
     kern = {k: v for k, v in prof.items() if k.startswith("k_")}
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dom]

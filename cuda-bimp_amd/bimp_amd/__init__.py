@@ -24,6 +24,7 @@ KP_MAX_POINTS = 8000
 OK, W_OVERFLOW, E_ARG, E_ALLOC, E_DEVICE, E_UNSUPPORTED = 0, 1, -1, -2, -3, -4
 MAP_DOUBLE, MAP_SINGLE, MAP_COMPLEX = 0, 1, 2
 FLAG_PROFILE = 1
+FLAG_SINGLE_STREAM = 2
 PRECISION_FP32, PRECISION_BF16, PRECISION_FP16 = 0, 1, 2
 DIRECT_MAX_KERNEL = 55
 

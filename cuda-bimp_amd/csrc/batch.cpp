@@ -253,6 +253,8 @@ int bimp_batch_create(const int* widths, const int* heights, int nframes, const 
     else
         bimp_default_options(&b->opt);
     b->opt.flags &= ~BIMP_FLAG_PROFILE;
+    // graph branches: one stream per context, no forked second stream inside a frame
+    b->opt.flags |= BIMP_FLAG_SINGLE_STREAM;
     std::map<std::pair<int, int>, int> idx;
     b->frameClass.resize(nframes);
     for (int f = 0; f < nframes; ++f) {

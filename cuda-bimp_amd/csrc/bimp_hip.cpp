@@ -76,7 +76,8 @@ struct LaunchScope {
 
 void collect_profile(bimp_ctx* c) {
     if (!c->evUsed) return;
-    (void)hipEventSynchronize(c->ev[c->evUsed - 1].b);
+    // launches on two streams complete out of record order: wait for every one
+    for (size_t i = 0; i < c->evUsed; ++i) (void)hipEventSynchronize(c->ev[i].b);
     for (size_t i = 0; i < c->evUsed; ++i) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, c->ev[i].a, c->ev[i].b) == hipSuccess) {
@@ -105,6 +106,11 @@ void free_all(bimp_ctx* c) {
     for (auto& kv : c->tw) (void)hipFree(kv.second);
     (void)hipFree(c->R);
     (void)hipFree(c->T);
+    (void)hipFree(c->R2);
+    (void)hipFree(c->T2);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->evFork) (void)hipEventDestroy(c->evFork);
+    if (c->evJoin) (void)hipEventDestroy(c->evJoin);
     (void)hipFree(c->maxslots);
     (void)hipFree(c->counter);
     (void)hipFree(c->blockCounts);
@@ -189,6 +195,16 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         lv[l] = c->level[l];
         lp[l] = (size_t)c->levelW[l];
     }
+    // FFT groups above level 0 go to the side stream once the pyramid is on s, except while
+    // s is being captured into a graph (there every group stays on s)
+    hipStreamCaptureStatus capturing = hipStreamCaptureStatusActive;
+    const bool useSide = c->side && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM) &&
+                         hipStreamIsCapturing(s, &capturing) == hipSuccess &&
+                         capturing == hipStreamCaptureStatusNone;
+    if (useSide) {
+        HIPCHK(c, hipEventRecord(c->evFork, s));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->evFork, 0));
+    }
     for (const Step& st : c->steps) {
         if (st.direct) {
             const Scale& s0 = c->sc[st.index];
@@ -220,6 +236,12 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         }
         const Group& gr = c->groups[st.index];
         const auto& g = c->sc[gr.first].g;
+        // groups above level 0 own R2/T2 whenever the context has a side stream, on s as well
+        const bool upper = c->side && g.level > 0;
+        const hipStream_t q = upper && useSide ? c->side : s;
+        float2* const R = upper ? c->R2 : c->R;
+        float2* const T = upper ? c->T2 : c->T;
+        const size_t tCap = upper ? c->tCap2 : c->tCap;
         const int ncols = g.FW / 2 + 1;
         const float2* twW = c->tw[g.FW];
         const float2* twH = c->tw[g.FH];
@@ -235,19 +257,19 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         const size_t rstride = (size_t)g.H * ncols;
         const double n = gr.n;
         {
-            LaunchScope ls(c, s, F_FWD_ROWS, n * ((double)g.H * g.W + 8.0 * g.H * ncols), g.FW);
+            LaunchScope ls(c, q, F_FWD_ROWS, n * ((double)g.H * g.W + 8.0 * g.H * ncols), g.FW);
             HIPCHK(c, bimpk::launch_fwd_rows(g.FW, lv[g.level], lp[g.level], g.H, g.W, ss,
-                                             c->opt.input_scale, c->R, ncols, rstride, twW, s));
+                                             c->opt.input_scale, R, ncols, rstride, twW, q));
         }
         {
-            LaunchScope ls(c, s, F_FWD_COLS, n * (8.0 * g.H * ncols + 8.0 * ncols * g.FH), g.FH);
-            HIPCHK(c, bimpk::launch_fwd_cols(g.FH, c->R, ncols, rstride, g.H, ss, ncols, twH, s));
+            LaunchScope ls(c, q, F_FWD_COLS, n * (8.0 * g.H * ncols + 8.0 * ncols * g.FH), g.FH);
+            HIPCHK(c, bimpk::launch_fwd_cols(g.FH, R, ncols, rstride, g.H, ss, ncols, twH, q));
         }
         // inverse passes over sub-batches of scales whose intermediate T fits the budget
         const Scale& s0 = c->sc[gr.first];
         const size_t tstride = (size_t)kNumOri * s0.Hc * g.FW;
         const int nb = std::max(1, std::min(c->invBatch > 0 ? c->invBatch : inverse_batch(tstride),
-                                            (int)(c->tCap / tstride)));
+                                            (int)(tCap / tstride)));
         for (int k0 = 0; k0 < gr.n; k0 += nb) {
             bimpk::ScaleSet sub;
             sub.n = std::min(nb, gr.n - k0);
@@ -260,26 +282,30 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
             }
             const double m = sub.n;
             {
-                LaunchScope ls(c, s, F_INV_COLS,
+                LaunchScope ls(c, q, F_INV_COLS,
                                m * (8.0 * ncols * g.FH + 4.0 * kUniqueOri * g.FW * g.FH +
                                     8.0 * kNumOri * s0.Hc * g.FW),
                                g.FH);
-                HIPCHK(c, bimpk::launch_inv_cols(g.FH, sub, g.FW, s0.Hc, c->T, tstride, twH, s));
+                HIPCHK(c, bimpk::launch_inv_cols(g.FH, sub, g.FW, s0.Hc, T, tstride, twH, q));
             }
             {
-                LaunchScope ls(c, s, F_INV_ROWS,
+                LaunchScope ls(c, q, F_INV_ROWS,
                                m * (8.0 * kNumOri * s0.Hc * g.FW + 4.0 * kNumOri * s0.Hc * s0.Wc),
                                g.FW);
-                HIPCHK(c, bimpk::launch_inv_rows(g.FW, sub, c->T, tstride, s0.Hc, s0.Wc, s0.ldc,
-                                                 twW, s));
+                HIPCHK(c, bimpk::launch_inv_rows(g.FW, sub, T, tstride, s0.Hc, s0.Wc, s0.ldc,
+                                                 twW, q));
             }
             if (cells) {
                 for (int k = 0; k < sub.n; ++k) {
-                    const int st = launch_cells_scale(c, (size_t)(gr.first + k0 + k), s);
+                    const int st = launch_cells_scale(c, (size_t)(gr.first + k0 + k), q);
                     if (st) return st;
                 }
             }
         }
+    }
+    if (useSide) {
+        HIPCHK(c, hipEventRecord(c->evJoin, c->side));
+        HIPCHK(c, hipStreamWaitEvent(s, c->evJoin, 0));
     }
     if (cells) c->maxValid = true;
     return BIMP_OK;
@@ -532,15 +558,25 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         c->steps.push_back({false, (int)c->groups.size() - 1});
         prevFft = true;
     }
-    size_t rMax = 0, tMax = 0;
+    size_t rMax[2] = {0, 0}, tMax[2] = {0, 0};
     std::vector<int> sizes;
     const unsigned hw = std::thread::hardware_concurrency();
     const int threads = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+    bool levels[2] = {false, false};
     for (const auto& gr : c->groups) {
         const Scale& s0 = c->sc[gr.first];
-        rMax = std::max(rMax, (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
+        const int k = s0.g.level > 0 ? 1 : 0;
+        levels[k] = true;
+        rMax[k] = std::max(rMax[k], (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
         const size_t ts = (size_t)kNumOri * s0.Hc * s0.g.FW;
-        tMax = std::max(tMax, ts * std::min(gr.n, inverse_batch(ts)));
+        tMax[k] = std::max(tMax[k], ts * std::min(gr.n, inverse_batch(ts)));
+    }
+    // a side stream only when both classes of FFT groups exist
+    const bool sideStream = levels[0] && levels[1] && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM);
+    if (!sideStream) {
+        rMax[0] = std::max(rMax[0], rMax[1]);
+        tMax[0] = std::max(tMax[0], tMax[1]);
+        rMax[1] = tMax[1] = 0;
     }
     bool anyDirect = false;
     for (auto& s : c->sc) {
@@ -592,9 +628,18 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         if (hipMemset(s.C, 0, (size_t)kNumOri * s.Hc * s.ldc * sizeof(float)) != hipSuccess)
             return bail(BIMP_E_DEVICE, "memset");
     }
-    c->tCap = tMax;
-    if ((rMax && dalloc(&c->R, rMax) != hipSuccess) || (tMax && dalloc(&c->T, tMax) != hipSuccess))
+    c->tCap = tMax[0];
+    c->tCap2 = tMax[1];
+    if ((rMax[0] && dalloc(&c->R, rMax[0]) != hipSuccess) ||
+        (tMax[0] && dalloc(&c->T, tMax[0]) != hipSuccess) ||
+        (rMax[1] && dalloc(&c->R2, rMax[1]) != hipSuccess) ||
+        (tMax[1] && dalloc(&c->T2, tMax[1]) != hipSuccess))
         return bail(BIMP_E_ALLOC, "scratch");
+    if (sideStream &&
+        (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
+        return bail(BIMP_E_DEVICE, "side stream");
     if (anyDirect) {
         if (hipError_t e = bimpk::prepare_direct(); e != hipSuccess)
             return bail(BIMP_E_DEVICE, std::string("kernel attributes: ") + hipGetErrorString(e));

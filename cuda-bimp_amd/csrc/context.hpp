@@ -99,6 +99,13 @@ struct bimp_ctx {
     std::map<std::string, bimpi::ProfAcc> prof;
     hipStream_t lastStream = nullptr;
     int invBatch = 0;  // inverse-pass sub-batch size; 0 = by the last-level-cache budget
+    // FFT groups above pyramid level 0 run on a side stream (forked after the pyramid, joined
+    // before the keypoint pass), with their own forward and inverse intermediates
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    float2* R2 = nullptr;
+    float2* T2 = nullptr;
+    size_t tCap2 = 0;
 };
 
 namespace bimpi {

@@ -75,3 +75,32 @@ def test_pitched_host_frame_equals_contiguous(B):
         assert st >= 0
         assert n.value == wn
         assert np.array_equal(out[:wn].view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("w,h", [(640, 480), (1920, 1080)])
+def test_two_streams_equal_one_stream(B, w, h):
+    """Transforms above pyramid level 0 on the context's second stream give the maps and
+    keypoints of the same frame run on one stream, bit for bit: with the flag set at creation
+    (no second stream made) and switched on and off at run time, on the null stream and on a
+    created one."""
+    import oracle_lib as O
+    from kpmatch import bits
+    img = O.synth_frame(w, h, 60, 8)
+    opt = B.default_options()
+    opt.flags = B.FLAG_SINGLE_STREAM
+    one = B.CudaStuff(w, h, O.BASE_BANK, opt)
+    want, wn = one.detect(img, cap=1 << 16)
+    wmaps = [(one.get_map(i, B.MAP_DOUBLE), one.get_map(i, B.MAP_SINGLE)) for i in range(len(one.ks))]
+    assert wn > 0
+    two = B.CudaStuff(w, h, O.BASE_BANK)
+    s = B.Stream()
+    for flags, stream in [(0, None), (0, s.handle), (B.FLAG_SINGLE_STREAM, s.handle), (0, s.handle)]:
+        two.set_flags(flags)
+        got, n = two.detect(img, cap=1 << 16, stream=stream)
+        assert n == wn
+        assert np.array_equal(bits(got), bits(want))
+        for i, (d, sm) in enumerate(wmaps):
+            assert np.array_equal(two.get_map(i, B.MAP_DOUBLE, stream=stream).view(np.uint32),
+                                  d.view(np.uint32))
+            assert np.array_equal(two.get_map(i, B.MAP_SINGLE, stream=stream).view(np.uint32),
+                                  sm.view(np.uint32))
