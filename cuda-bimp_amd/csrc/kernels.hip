@@ -327,23 +327,36 @@ struct Contrib {
     float s, d, t, r;
 };
 
-// Contributions of orientation o at one pixel: m = its own tile, m2 = the tile of (o+4)%8.
-__device__ __forceinline__ Contrib cell_terms(const float* __restrict__ m,
-                                              const float* __restrict__ m2,
-                                              const CellTileOffsets& q, int o) {
+// Contributions of orientation o at a thread's kCellRows pixels (tile elements li + rr*RS):
+// m = its own staged tile, m2 = the tile of (o+4)%8.  Each tap's pointer is uniform, so one
+// address add serves every row and the row step folds into the LDS instruction's offset.
+template <int RS>
+__device__ __forceinline__ void cell_terms(const float* __restrict__ m,
+                                           const float* __restrict__ m2,
+                                           const CellTileOffsets& q, int o, int li,
+                                           Contrib (&c)[kCellRows]) {
 #pragma clang fp contract(off)
-    // Products by 0.5 and 16 are exact, so fma(-0.5, a, c) and fma(-16, m, c) round once
-    // exactly where the reference's separate multiply and subtract do: same bits, fewer ops.
-    Contrib c;
-    const float centre = m[0];
-    c.s = fmaxf(m[q.es1[o]] - m[q.es2[o]], 0.f);
-    const double d1 = __builtin_fma(-0.5, (double)m[q.ed1[o]], (double)centre);
-    c.d = fmaxf((float)__builtin_fma(-0.5, (double)m[q.ed2[o]], d1), 0.f);
-    c.t = fmaxf(m[q.tn[o]] - centre, 0.f);
-    float v = fmaxf(__builtin_fmaf(-16.f, m2[q.rp[o]], centre), 0.f);
-    v += __builtin_fmaf(-16.f, m2[q.rm[o]], centre);
-    c.r = fmaxf(v, 0.f);
-    return c;
+    const float* __restrict__ s1 = m + q.es1[o];
+    const float* __restrict__ s2 = m + q.es2[o];
+    const float* __restrict__ d1p = m + q.ed1[o];
+    const float* __restrict__ d2p = m + q.ed2[o];
+    const float* __restrict__ tn = m + q.tn[o];
+    const float* __restrict__ rp = m2 + q.rp[o];
+    const float* __restrict__ rm = m2 + q.rm[o];
+#pragma unroll
+    for (int rr = 0; rr < kCellRows; ++rr) {
+        const int l = li + rr * RS;
+        // Products by 0.5 and 16 are exact, so fma(-0.5, a, c) and fma(-16, m, c) round once
+        // exactly where the reference's separate multiply and subtract do: same bits, fewer ops.
+        const float centre = m[l];
+        c[rr].s = fmaxf(s1[l] - s2[l], 0.f);
+        const double d1 = __builtin_fma(-0.5, (double)d1p[l], (double)centre);
+        c[rr].d = fmaxf((float)__builtin_fma(-0.5, (double)d2p[l], d1), 0.f);
+        c[rr].t = fmaxf(tn[l] - centre, 0.f);
+        float v = fmaxf(__builtin_fmaf(-16.f, rp[l], centre), 0.f);
+        v += __builtin_fmaf(-16.f, rm[l], centre);
+        c[rr].r = fmaxf(v, 0.f);
+    }
 }
 
 template <int HS>
@@ -364,7 +377,7 @@ __device__ __forceinline__ void cells_fetch(const MapView& mv, int H, int W, int
     for (int k = 0; k < CG::PER; ++k) {
         const int i = tid + k * kCellTW * kCellTY;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < CG::NV4) {
+        if ((k + 1) * kCellTW * kCellTY <= CG::NV4 || i < CG::NV4) {
             const int row = i / CG::LW4, c4 = i - row * CG::LW4;
             const int m = row >= CG::LH ? 1 : 0;
             const int ry = row - m * CG::LH;
@@ -385,6 +398,44 @@ __device__ __forceinline__ void cells_fetch(const MapView& mv, int H, int W, int
     }
 }
 
+// Interior tiles: byte offsets, from the map of the pair's first orientation, of the float4s
+// this thread stages (the pair's second orientation lies 4 maps further on).  Computed once
+// per tile; a pair then adds only a uniform 64-bit base, and the loads take the
+// scalar-base + 32-bit-offset form.  The launcher keeps 8 maps' bytes below 2^32.
+template <int HS>
+__device__ __forceinline__ void cells_offsets(const MapView& mv, int x0, int y0, int tid,
+                                              unsigned (&off)[CellGeom<HS>::PER]) {
+    using CG = CellGeom<HS>;
+#pragma unroll
+    for (int k = 0; k < CG::PER; ++k) {
+        const int i = tid + k * kCellTW * kCellTY;
+        const int row = i / CG::LW4, c4 = i - row * CG::LW4;
+        const int m = row >= CG::LH ? 1 : 0;
+        const int gy = y0 - HS + row - m * CG::LH, gx = x0 - HS + 4 * c4;
+        // interior tiles: gy, gx >= 0
+        off[k] = i < CG::NV4 ? ((unsigned)(4 * m * mv.Hc + gy) * (unsigned)mv.ldc + (unsigned)gx) *
+                                   (unsigned)sizeof(float)
+                             : 0u;
+    }
+}
+
+template <int HS>
+__device__ __forceinline__ void cells_fetch_interior(const MapView& mv, int pr, int tid,
+                                                     const unsigned (&off)[CellGeom<HS>::PER],
+                                                     float4 (&r)[CellGeom<HS>::PER]) {
+    using CG = CellGeom<HS>;
+    const char* base = reinterpret_cast<const char*>(mv.base + (size_t)pr * mv.Hc * mv.ldc);
+#pragma unroll
+    for (int k = 0; k < CG::PER; ++k) {
+        const int i = tid + k * kCellTW * kCellTY;
+        // every thread stages a float4 for k with (k+1) * NT <= NV4
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if ((k + 1) * kCellTW * kCellTY <= CG::NV4 || i < CG::NV4)
+            v = *reinterpret_cast<const float4*>(base + off[k]);
+        r[k] = v;
+    }
+}
+
 template <int HS>
 __device__ __forceinline__ void cells_put(float* __restrict__ tile, int tid,
                                           const float4 (&r)[CellGeom<HS>::PER]) {
@@ -392,7 +443,8 @@ __device__ __forceinline__ void cells_put(float* __restrict__ tile, int tid,
 #pragma unroll
     for (int k = 0; k < CG::PER; ++k) {
         const int i = tid + k * kCellTW * kCellTY;
-        if (i < CG::NV4) reinterpret_cast<float4*>(tile)[i] = r[k];
+        if ((k + 1) * kCellTW * kCellTY <= CG::NV4 || i < CG::NV4)
+            reinterpret_cast<float4*>(tile)[i] = r[k];
     }
 }
 
@@ -412,21 +464,35 @@ __device__ __forceinline__ void cells_tile(const MapView& mv, int H, int W,
 #pragma unroll
     for (int rr = 0; rr < kCellRows; ++rr) accS[rr] = accD[rr] = accT[rr] = accR[rr] = 0.f;
     float4 pre[CG::PER];
-    cells_fetch<HS, INTERIOR>(mv, H, W, x0, y0, 0, tid, pre);
+    unsigned off[CG::PER];
+    if (INTERIOR) {
+        cells_offsets<HS>(mv, x0, y0, tid, off);
+        cells_fetch_interior<HS>(mv, 0, tid, off, pre);
+    } else {
+        cells_fetch<HS, false>(mv, H, W, x0, y0, 0, tid, pre);
+    }
     cells_put<HS>(tile, tid, pre);
     __syncthreads();
+    // opaque to the compiler, so that its constant part is not split off into LDS offsets
+    // (which the row-pair reads cannot hold)
+    int li = (threadIdx.y + HS) * CG::LW + (threadIdx.x + HS);
+    asm volatile("" : "+v"(li));
 #pragma unroll
     for (int pr = 0; pr < 4; ++pr) {
-        if (pr < 3) cells_fetch<HS, INTERIOR>(mv, H, W, x0, y0, pr + 1, tid, pre);
+        if (pr < 3) {
+            if (INTERIOR)
+                cells_fetch_interior<HS>(mv, pr + 1, tid, off, pre);
+            else
+                cells_fetch<HS, false>(mv, H, W, x0, y0, pr + 1, tid, pre);
+        }
         const int o1 = pr, o2 = pr + 4;
+        Contrib cc1[kCellRows], cc2[kCellRows];
+        cell_terms<kCellTY * CG::LW>(tile, tile + CG::TS, q, o1, li, cc1);
+        cell_terms<kCellTY * CG::LW>(tile + CG::TS, tile, q, o2, li, cc2);
 #pragma unroll
         for (int rr = 0; rr < kCellRows; ++rr) {
             const int y = y0 + threadIdx.y + kCellTY * rr;
-            const int li = (y - y0 + HS) * CG::LW + (x - x0 + HS);
-            const float* m1 = tile + li;
-            const float* m2 = tile + CG::TS + li;
-            Contrib c1 = cell_terms(m1, m2, q, o1);
-            Contrib c2 = cell_terms(m2, m1, q, o2);
+            Contrib c1 = cc1[rr], c2 = cc2[rr];
             if (!INTERIOR) {
                 const bool e1 = x >= q.ex[o1] && x <= W - q.ex[o1] && y >= q.ey[o1] && y <= H - q.ey[o1];
                 const bool i1 = x >= q.ix[o1] && x <= W - q.ix[o1] && y >= q.iy[o1] && y <= H - q.iy[o1];
@@ -998,7 +1064,9 @@ static hipError_t launch_cells_hs(const MapView& mv, int FH, int FW, int H, int 
     tg.ix1 = std::max(tg.ix0, std::min(tg.tiles_x, end(W - HS - (kCellTW - 1), kCellTW)));
     tg.iy0 = first(HS, kCellTH);
     tg.iy1 = std::max(tg.iy0, std::min(tg.tiles_y, end(H - HS - (kCellTH - 1), kCellTH)));
-    if (W >= FW || H >= FH || tg.ix0 >= tg.tiles_x || tg.iy0 >= tg.tiles_y)
+    // (interior staging offsets span 5 maps from the pair's base: 8 maps must fit 32 bits)
+    if (W >= FW || H >= FH || tg.ix0 >= tg.tiles_x || tg.iy0 >= tg.tiles_y ||
+        (size_t)kNumOri * mv.Hc * mv.ldc * sizeof(float) > 0xffffffffu)
         tg.ix0 = tg.ix1 = tg.iy0 = tg.iy1 = 0;  // everything takes the checked path
     const int nin = (tg.ix1 - tg.ix0) * (tg.iy1 - tg.iy0);
     const int nbd = tg.tiles_x * tg.tiles_y - nin;
