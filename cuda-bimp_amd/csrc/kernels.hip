@@ -315,12 +315,15 @@ constexpr int kCellTW = 64, kCellTH = 16, kCellTY = 8, kCellRows = kCellTH / kCe
 // dynamic LDS limit per workgroup (160 KiB per CU, minus room for static LDS)
 
 struct CellTileOffsets {
-    // LDS element offsets relative to the pixel's own position in a staged tile
-    int es1[kNumOri], es2[kNumOri], ed1[kNumOri], ed2[kNumOri];  // (-a,b) (a,-b) (-2a,2b) (2a,-2b)
-    int tn[kNumOri];                                              // (-a',-b')
-    int rp[kNumOri], rm[kNumOri];                                 // partner map at +-(a'',b'')
-    // validity: 2|b| <= x <= W - 2|b| and 2|a| <= y <= H - 2|a| (end-stop, inhibition)
-    int ex[kNumOri], ey[kNumOri], ix[kNumOri], iy[kNumOri];
+    // LDS element offsets relative to the pixel's own position in a staged tile.  The
+    // end-stop taps are +-es and +-2 es, es = (-a,b); the radial taps are +-rp on the partner
+    // map.  Only the independent ones are passed: fewer scalar registers.
+    int es[kNumOri];  // (-a,b)
+    int tn[kNumOri];  // (-a',-b')
+    int rp[kNumOri];  // (a'',b'')
+    // validity: 2|b| <= x <= W - 2|b| and 2|a| <= y <= H - 2|a| (end-stop, inhibition), the
+    // x bound in the low and the y bound in the high 16 bits
+    int exy[kNumOri], ixy[kNumOri];
 };
 
 struct Contrib {
@@ -336,13 +339,14 @@ __device__ __forceinline__ void cell_terms(const float* __restrict__ m,
                                            const CellTileOffsets& q, int o, int li,
                                            Contrib (&c)[kCellRows]) {
 #pragma clang fp contract(off)
-    const float* __restrict__ s1 = m + q.es1[o];
-    const float* __restrict__ s2 = m + q.es2[o];
-    const float* __restrict__ d1p = m + q.ed1[o];
-    const float* __restrict__ d2p = m + q.ed2[o];
+    const int es = q.es[o], r = q.rp[o];
+    const float* __restrict__ s1 = m + es;
+    const float* __restrict__ s2 = m - es;
+    const float* __restrict__ d1p = m + 2 * es;
+    const float* __restrict__ d2p = m - 2 * es;
     const float* __restrict__ tn = m + q.tn[o];
-    const float* __restrict__ rp = m2 + q.rp[o];
-    const float* __restrict__ rm = m2 + q.rm[o];
+    const float* __restrict__ rp = m2 + r;
+    const float* __restrict__ rm = m2 - r;
 #pragma unroll
     for (int rr = 0; rr < kCellRows; ++rr) {
         const int l = li + rr * RS;
@@ -494,10 +498,12 @@ __device__ __forceinline__ void cells_tile(const MapView& mv, int H, int W,
             const int y = y0 + threadIdx.y + kCellTY * rr;
             Contrib c1 = cc1[rr], c2 = cc2[rr];
             if (!INTERIOR) {
-                const bool e1 = x >= q.ex[o1] && x <= W - q.ex[o1] && y >= q.ey[o1] && y <= H - q.ey[o1];
-                const bool i1 = x >= q.ix[o1] && x <= W - q.ix[o1] && y >= q.iy[o1] && y <= H - q.iy[o1];
-                const bool e2 = x >= q.ex[o2] && x <= W - q.ex[o2] && y >= q.ey[o2] && y <= H - q.ey[o2];
-                const bool i2 = x >= q.ix[o2] && x <= W - q.ix[o2] && y >= q.iy[o2] && y <= H - q.iy[o2];
+                auto inside = [&](int b) {
+                    const int bx = b & 0xffff, by = b >> 16;
+                    return x >= bx && x <= W - bx && y >= by && y <= H - by;
+                };
+                const bool e1 = inside(q.exy[o1]), i1 = inside(q.ixy[o1]);
+                const bool e2 = inside(q.exy[o2]), i2 = inside(q.ixy[o2]);
                 if (!e1) c1.s = c1.d = 0.f;
                 if (!i1) c1.t = c1.r = 0.f;
                 if (!e2) c2.s = c2.d = 0.f;
@@ -1041,17 +1047,12 @@ static hipError_t launch_cells_hs(const MapView& mv, int FH, int FW, int H, int 
     CellTileOffsets q;
     for (int o = 0; o < kNumOri; ++o) {
         const int a = p.ea[o], b = p.eb[o];
-        q.es1[o] = -a * CG::LW + b;
-        q.es2[o] = a * CG::LW - b;
-        q.ed1[o] = -2 * a * CG::LW + 2 * b;
-        q.ed2[o] = 2 * a * CG::LW - 2 * b;
+        q.es[o] = -a * CG::LW + b;
         q.tn[o] = -p.ia[o] * CG::LW - p.ib[o];
         q.rp[o] = p.ra[o] * CG::LW + p.rb[o];
-        q.rm[o] = -p.ra[o] * CG::LW - p.rb[o];
-        q.ex[o] = 2 * abs(b);
-        q.ey[o] = 2 * abs(a);
-        q.ix[o] = 2 * abs(p.ib[o]);
-        q.iy[o] = 2 * abs(p.ia[o]);
+        // (bounds are at most twice an offset within the halo: far below 2^15)
+        q.exy[o] = 2 * abs(b) | (2 * abs(a)) << 16;
+        q.ixy[o] = 2 * abs(p.ib[o]) | (2 * abs(p.ia[o])) << 16;
     }
     dim3 block(kCellTW, kCellTY);
     CellTiles tg;
