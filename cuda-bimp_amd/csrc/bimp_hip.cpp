@@ -700,7 +700,7 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         tab.push_back(d);
     }
     c->totalBlocks = blocks;
-    if (dalloc(&c->nmsTable, tab.size()) != hipSuccess || dalloc(&c->blockCounts, blocks) != hipSuccess ||
+    if (dalloc(&c->nmsTable, tab.size()) != hipSuccess || dalloc(&c->blockCounts, blocks + 1) != hipSuccess ||
         dalloc(&c->maxslots, c->sc.size()) != hipSuccess || dalloc(&c->counter, 1) != hipSuccess)
         return bail(BIMP_E_ALLOC, "nms buffers");
     if (hipMemcpy(c->nmsTable, tab.data(), tab.size() * sizeof(bimpk::NmsScale),

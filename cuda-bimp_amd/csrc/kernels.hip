@@ -642,13 +642,12 @@ __device__ __forceinline__ int find_scale(const NmsScale* sc, int nscales, int b
 
 // identifyKeypoints_kernel predicate (cuda_bimp.cuh:100-108): exclusive border `off`,
 // D - max8(D) > 0 (dilate with the centre removed, cuda_bimp.cu:73-85, :733), D > thresh.
-__device__ __forceinline__ bool nms_hit(const float* D, int H, int W, int off, float thresh,
-                                        int y, int x) {
+// Evaluated as D > thresh first (one load, rarely true), then the rest:
+__device__ __forceinline__ bool nms_rest(const float* D, int H, int W, int off, float v, int y,
+                                         int x) {
 #pragma clang fp contract(off)
     if (!(x > off && x < W - off && y > off && y < H - off)) return false;
     const float* p = D + (size_t)y * W;
-    const float v = p[x];
-    if (!(v > thresh)) return false;
     const float* pu = p - W;
     const float* pd = p + W;
     float nb = fmaxf(pu[x - 1], pu[x]);
@@ -659,6 +658,29 @@ __device__ __forceinline__ bool nms_hit(const float* D, int H, int W, int off, f
     nb = fmaxf(nb, pd[x]);
     nb = fmaxf(nb, pd[x + 1]);
     return v - nb > 0.f;
+}
+
+// Hit mask of a thread's kNmsPerThread pixels (bit i: pixel base + i*kNmsThreads + tid).
+// All the pixels' loads go out first; the row/column split and the neighbourhood are only
+// computed for the few above the threshold.
+__device__ __forceinline__ unsigned nms_mask(const NmsScale& d, float thresh, int base) {
+    const int A = d.H * d.W;
+    float v[kNmsPerThread];
+#pragma unroll
+    for (int i = 0; i < kNmsPerThread; ++i) {
+        const int pix = base + i * kNmsThreads + (int)threadIdx.x;
+        v[i] = pix < A ? d.D[pix] : 0.f;
+    }
+    unsigned mask = 0;
+#pragma unroll
+    for (int i = 0; i < kNmsPerThread; ++i) {
+        const int pix = base + i * kNmsThreads + (int)threadIdx.x;
+        if (pix < A && v[i] > thresh) {
+            const int y = pix / d.W, x = pix - y * d.W;
+            if (nms_rest(d.D, d.H, d.W, d.off, v[i], y, x)) mask |= 1u << i;
+        }
+    }
+    return mask;
 }
 
 __device__ __forceinline__ float scale_thresh(const unsigned* maxslots, int slot, double thr) {
@@ -674,13 +696,8 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_count(const NmsScale* __res
     const int s = find_scale(sc, nscales, blk);
     const NmsScale d = sc[s];
     const float thresh = scale_thresh(maxslots, d.slot, thr);
-    const int A = d.H * d.W;
     const int base = (blk - d.block0) * kNmsBlockPixels;
-    int c = 0;
-    for (int i = 0; i < kNmsPerThread; ++i) {
-        const int pix = base + i * kNmsThreads + threadIdx.x;
-        if (pix < A) c += nms_hit(d.D, d.H, d.W, d.off, thresh, pix / d.W, pix % d.W) ? 1 : 0;
-    }
+    int c = __popc(nms_mask(d, thresh, base));
 #pragma unroll
     for (int sh = 32; sh >= 1; sh >>= 1) c += __shfl_xor(c, sh, 64);
     __shared__ int wc[kNmsThreads / 64];
@@ -693,7 +710,8 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_count(const NmsScale* __res
     }
 }
 
-// Exclusive scan of the per-block counts in place (one workgroup); the total goes to counter.
+// Exclusive scan of the per-block counts in place (one workgroup); the total goes to counter
+// and to counts[n].
 __global__ void __launch_bounds__(1024) k_nms_scan(int* __restrict__ counts, int n,
                                                    int* __restrict__ counter) {
     __shared__ int part[1024];
@@ -716,7 +734,10 @@ __global__ void __launch_bounds__(1024) k_nms_scan(int* __restrict__ counts, int
         counts[i] = run;
         run += c;
     }
-    if (tid == 1023) *counter = part[1023];
+    if (tid == 1023) {
+        *counter = part[1023];
+        counts[n] = part[1023];
+    }
 }
 
 // Re-evaluates the predicate and writes each keypoint at its row-major rank.  matrix_ld > 0
@@ -729,28 +750,41 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
     const int blk = blockIdx.x;
     const int s = find_scale(sc, nscales, blk);
     const NmsScale d = sc[s];
+    const int start = offsets[blk];
+    // a block without keypoints (its count from the scan), or whose first rank is past the
+    // capacity, writes nothing
+    if (offsets[blk + 1] == start || start >= cap) return;
     const float thresh = scale_thresh(maxslots, d.slot, thr);
-    const int A = d.H * d.W;
     const int base = (blk - d.block0) * kNmsBlockPixels;
-    __shared__ int wc[kNmsThreads / 64];
-    int run = offsets[blk];
+    constexpr int NW = kNmsThreads / 64;
+    // rank of pixel (i, thread) = keypoints of earlier iterations, then of earlier waves in
+    // iteration i, then of earlier lanes: the row-major order of the pixels
+    __shared__ int wc[kNmsPerThread * NW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned mask = nms_mask(d, thresh, base);
+    unsigned rank_lo[kNmsPerThread];
+#pragma unroll
     for (int i = 0; i < kNmsPerThread; ++i) {
-        const int pix = base + i * kNmsThreads + threadIdx.x;
-        const int y = pix / d.W, x = pix % d.W;
-        const bool hit = pix < A && nms_hit(d.D, d.H, d.W, d.off, thresh, y, x);
-        const unsigned long long bal = __ballot(hit);
-        const int rank_in_wave = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wc[wave] = __popcll(bal);
-        __syncthreads();
-        int before = 0, total = 0;
-        for (int w = 0; w < kNmsThreads / 64; ++w) {
-            before += w < wave ? wc[w] : 0;
-            total += wc[w];
+        const unsigned long long bal = __ballot((mask >> i) & 1u);
+        rank_lo[i] = (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wc[i * NW + wave] = __popcll(bal);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = start;
+        for (int j = 0; j < kNmsPerThread * NW; ++j) {
+            const int c = wc[j];
+            wc[j] = run;
+            run += c;
         }
-        __syncthreads();
-        if (hit) {
-            const int idx = run + before + rank_in_wave;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kNmsPerThread; ++i) {
+        if ((mask >> i) & 1u) {
+            const int pix = base + i * kNmsThreads + (int)threadIdx.x;
+            const int y = pix / d.W, x = pix - y * d.W;
+            const int idx = wc[i * NW + wave] + (int)rank_lo[i];
             if (idx < cap) {
                 const float* p = d.D + (size_t)y * d.W;
                 const float xv1 = p[x - 1], xv2 = p[x], xv3 = p[x + 1];
@@ -785,7 +819,6 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
                 }
             }
         }
-        run += total;
     }
 }
 
