@@ -17,6 +17,7 @@
 // Twiddles come from a per-N packed table (TwLayout) computed in FP64 on the host and
 // rounded to FP32; the inverse transform uses their conjugates (unnormalised).
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 namespace bimpdev {
@@ -229,9 +230,15 @@ __device__ __forceinline__ void stage_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int N, bool INV, int Ns, int R>
-__device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
-                                            const float2* __restrict__ tw) {
+// Twiddles copied into LDS by a workgroup that runs many transforms (tw_to_lds below): each
+// stage's Ns rows of P/2 float4 are stored transposed, float4 q of row k at q*Ns + k, so the
+// lanes of a wavefront (consecutive k) read consecutive float4s.
+struct LdsTw {
+    const __attribute__((address_space(3))) float4* p;
+};
+
+template <int N, bool INV, int Ns, int R, class TW>
+__device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane, TW tw) {
     using C = FFTCfg<N>;
     constexpr int P = tw_row(R);
 #pragma unroll
@@ -239,10 +246,16 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
         if constexpr (Ns > 1) {
             const int j = lane + C::TPV * m;
             const int k = j % Ns;
-            const float4* row = reinterpret_cast<const float4*>(tw + k * P);
             float4 t[P / 2];
+            if constexpr (std::is_same<TW, LdsTw>::value) {
 #pragma unroll
-            for (int q = 0; q < P / 2; ++q) t[q] = row[q];
+                for (int q = 0; q < P / 2; ++q)
+                    t[q] = *(const float4*)(tw.p + q * Ns + k);  // generic: address space inferred
+            } else {
+                const float4* row = reinterpret_cast<const float4*>(tw + k * P);
+#pragma unroll
+                for (int q = 0; q < P / 2; ++q) t[q] = row[q];
+            }
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 const float4 tq = t[(r - 1) / 2];
@@ -317,17 +330,55 @@ struct Stages;
 template <int N, bool INV, int Ns, int R, int... Rest>
 struct Stages<N, INV, Ns, RadList<R, Rest...>> {
     // tw points at this stage's rows (unused for the first stage)
+    template <class TW>
     static __device__ __forceinline__ void run(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
-                                               const float2* __restrict__ tw) {
+                                               TW tw) {
         butterflies<N, INV, Ns, R>(v, lane, tw);
         if constexpr (sizeof...(Rest) > 0) {
             exchange<N, Ns, R, RadInfo<RadList<Rest...>>::first>(v, lds, lane);
             constexpr int next_ns = Ns * R;
-            constexpr int adv = Ns > 1 ? Ns * tw_row(R) : 0;
-            Stages<N, INV, next_ns, RadList<Rest...>>::run(v, lds, lane, tw + adv);
+            constexpr int adv = Ns > 1 ? Ns * tw_row(R) : 0;  // float2
+            if constexpr (std::is_same<TW, LdsTw>::value)
+                Stages<N, INV, next_ns, RadList<Rest...>>::run(v, lds, lane, LdsTw{tw.p + adv / 2});
+            else
+                Stages<N, INV, next_ns, RadList<Rest...>>::run(v, lds, lane, tw + adv);
         }
     }
 };
+
+// Copies the packed table of FFTCfg<N> (global, row-major) into LDS in the transposed layout
+// LdsTw reads; tw_lds_bytes<N>() bytes.  Threads tid = 0..nthr-1 of the workgroup share it.
+template <class L, int Ns = 1, bool First = true>
+struct TwCopy;
+template <int R, int... Rest, int Ns, bool First>
+struct TwCopy<RadList<R, Rest...>, Ns, First> {
+    static __device__ __forceinline__ void run(const float4* __restrict__ src, float4* dst,
+                                               int tid, int nthr) {
+        if constexpr (!First) {
+            constexpr int P4 = tw_row(R) / 2;
+            for (int i = tid; i < Ns * P4; i += nthr) {
+                const int k = i / P4, q = i - k * P4;
+                dst[q * Ns + k] = src[i];
+            }
+            TwCopy<RadList<Rest...>, Ns * R, false>::run(src + Ns * P4, dst + Ns * P4, tid, nthr);
+        } else {
+            TwCopy<RadList<Rest...>, Ns * R, false>::run(src, dst, tid, nthr);
+        }
+    }
+};
+template <int Ns, bool First>
+struct TwCopy<RadList<>, Ns, First> {
+    static __device__ __forceinline__ void run(const float4*, float4*, int, int) {}
+};
+
+template <int N>
+__host__ __device__ constexpr size_t tw_lds_bytes() {
+    return (size_t)TwLayout<typename FFTCfg<N>::Rad>::size * sizeof(float2);
+}
+template <int N>
+__device__ __forceinline__ void tw_to_lds(const float2* tw, float4* dst, int tid, int nthr) {
+    TwCopy<typename FFTCfg<N>::Rad>::run(reinterpret_cast<const float4*>(tw), dst, tid, nthr);
+}
 
 // The twiddle pointer is passed through an empty asm statement so that a kernel running
 // several transforms in a loop reloads twiddles (L1 hits) instead of keeping every one of
@@ -336,6 +387,11 @@ template <int N, bool INV>
 __device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
                                     const float2* tw) {
     asm volatile("" : "+s"(tw));
+    Stages<N, INV, 1, typename FFTCfg<N>::Rad>::run(v, lds, lane, tw);
+}
+template <int N, bool INV>
+__device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane, LdsTw tw) {
+    asm volatile("" : "+s"(tw.p));
     Stages<N, INV, 1, typename FFTCfg<N>::Rad>::run(v, lds, lane, tw);
 }
 

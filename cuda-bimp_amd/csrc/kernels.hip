@@ -174,7 +174,12 @@ struct InvColCfg {
     static constexpr int SS = BR + 2;  // staging stride (float2); 2 (mod 16): conflict-free reads
     static constexpr size_t STAGE = (size_t)CB * SS * sizeof(float2);
     static constexpr size_t EXCH = (size_t)CB * fft_lds_floats<FH>() * sizeof(float);
-    static constexpr size_t LDS = STAGE > EXCH ? STAGE : EXCH;
+    static constexpr size_t MAIN = STAGE > EXCH ? STAGE : EXCH;
+    // The persistent workgroup keeps the twiddle table in LDS (after the exchange/staging
+    // region) when that still leaves room for two workgroups per CU: LDS reads instead of
+    // cache round trips inside each stage.
+    static constexpr bool LDS_TW = MAIN + tw_lds_bytes<FH>() <= 80u * 1024u;
+    static constexpr size_t LDS = MAIN + (LDS_TW ? tw_lds_bytes<FH>() : 0);
 };
 
 template <int FH>
@@ -188,6 +193,11 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
     const int half = FW / 2;
     const int g = threadIdx.x / C::TPV, lane0 = threadIdx.x % C::TPV;
     float* my = reinterpret_cast<float*>(lds2) + (size_t)g * fft_lds_floats<FH>();
+    float4* twl = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds2) + IC::MAIN);
+    if constexpr (IC::LDS_TW) {
+        tw_to_lds<FH>(tw, twl, threadIdx.x, IC::THREADS);
+        __syncthreads();
+    }
     // A unit is one pass (direct or partner) of one orientation of one column block.  The
     // grid is at most one resident round of workgroups and walks the units with stride
     // gridDim.x (a multiple of 8), so the work splits evenly in half-block units instead of
@@ -222,7 +232,11 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
             const int kg = (partner && ky) ? FH - ky : ky;
             v[e] = cscale(dcol[ky], gcol[kg]);
         }
-        fft<FH, true>(v, my, lane, tw);
+        if constexpr (IC::LDS_TW)
+            fft<FH, true>(v, my, lane,
+                          LdsTw{(const __attribute__((address_space(3))) float4*)twl});
+        else
+            fft<FH, true>(v, my, lane, tw);
         if (partner) {
 #pragma unroll
             for (int e = 0; e < C::E; ++e) v[e].y = -v[e].y;
