@@ -295,6 +295,59 @@ __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __re
     }
 }
 
+// Persistent form for transforms of one full wavefront (TPV = 64) up to 2048 points: each
+// workgroup copies the twiddle table into LDS once and its WPB wavefronts then walk units of
+// WPB consecutive rows (one row per wavefront) of one orientation of one scale, stride
+// gridDim.x.  Same arithmetic as k_inv_rows.
+template <int FW>
+struct InvRowCfg {
+    static constexpr int WPB = 4;
+    static constexpr int THREADS = 64 * WPB;
+    static constexpr size_t MAIN = (size_t)WPB * fft_lds_floats<FW>() * sizeof(float);
+    static constexpr size_t LDS = MAIN + tw_lds_bytes<FW>();
+    static constexpr bool PERSISTENT = FFTCfg<FW>::TPV == 64 && FW <= 2048;
+};
+
+template <int FW>
+__global__ void __launch_bounds__(InvRowCfg<FW>::THREADS) k_inv_rows_p(
+        ScaleSet ss, const float2* __restrict__ Tbase, size_t tstride, int Hc, int Wc, int ldc,
+        int units, const float2* __restrict__ tw) {
+    using C = FFTCfg<FW>;
+    using RC = InvRowCfg<FW>;
+    static_assert(C::TPV == 64, "one row per wavefront");
+    extern __shared__ float lds[];
+    const int w = threadIdx.x / 64, lane0 = threadIdx.x % 64;
+    float* my = lds + (size_t)w * fft_lds_floats<FW>();
+    float4* twl = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds) + RC::MAIN);
+    tw_to_lds<FW>(tw, twl, threadIdx.x, RC::THREADS);
+    __syncthreads();
+    const int nrb = (Hc + RC::WPB - 1) / RC::WPB;
+#pragma unroll 1
+    for (int u = blockIdx.x; u < units; u += gridDim.x) {
+        const int rb = u % nrb, o = (u / nrb) % kNumOri, sidx = u / (nrb * kNumOri);
+        // lane-derived indices are recomputed per unit rather than kept live across units
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        const int y = rb * RC::WPB + w;
+        const bool active = y < Hc;
+        const float2* row =
+            Tbase + (size_t)sidx * tstride + ((size_t)o * Hc + (active ? y : 0)) * FW;
+        float2 v[C::E];
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) v[e] = row[in_index<FW>(e, lane)];
+        fft<FW, true>(v, my, lane, LdsTw{(const __attribute__((address_space(3))) float4*)twl});
+        if (active) {
+            float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
+#pragma unroll
+            for (int e = 0; e < C::E; ++e) {
+                const int x = out_index<FW>(e, lane);
+                // native v_sqrt_f32 (1 ulp): the map tolerance is 1e-4 relative
+                if (x < Wc) out[x] = __builtin_amdgcn_sqrtf(v[e].x * v[e].x + v[e].y * v[e].y);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ cells
 __device__ __forceinline__ unsigned float_key(float f) {
     const unsigned u = __float_as_uint(f);
@@ -967,6 +1020,29 @@ static int inv_cols_slots() {
     return slots;
 }
 
+// Resident workgroups of k_inv_rows_p<N> on the whole device (one round), a multiple of 8.
+template <int N>
+static int inv_rows_slots() {
+    static int slots = [] {
+        if constexpr (!InvRowCfg<N>::PERSISTENT) {
+            return 0;
+        } else {
+            int dev = 0, cus = 0, per = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipFuncSetAttribute((const void*)k_inv_rows_p<N>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)InvRowCfg<N>::LDS) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_inv_rows_p<N>,
+                                                             InvRowCfg<N>::THREADS,
+                                                             InvRowCfg<N>::LDS) != hipSuccess)
+                return 0;
+            return std::max(1, cus * per / 8) * 8;
+        }
+    }();
+    return slots;
+}
+
 template <int N>
 static hipError_t prepare_n() {
     hipError_t e;
@@ -985,6 +1061,7 @@ static hipError_t prepare_n() {
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds)) !=
         hipSuccess)
         return e;
+    if (InvRowCfg<N>::PERSISTENT && inv_rows_slots<N>() <= 0) return hipErrorInvalidValue;
     return inv_cols_slots<N>() > 0 ? hipSuccess : hipErrorInvalidValue;
 }
 
@@ -1074,6 +1151,15 @@ hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t t
                            int Wc, int ldc, const float2* tw, hipStream_t s) {
 #define X(N)                                                                                 \
     if (FW == N) {                                                                           \
+        if constexpr (InvRowCfg<N>::PERSISTENT) {                                            \
+            using RC = InvRowCfg<N>;                                                         \
+            const int units = (Hc + RC::WPB - 1) / RC::WPB * kNumOri * ss.n;                 \
+            const int grid = std::min(units, inv_rows_slots<N>());                           \
+            if (grid <= 0) return hipErrorInvalidValue;                                      \
+            k_inv_rows_p<N><<<grid, RC::THREADS, RC::LDS, s>>>(ss, T, tstride, Hc, Wc, ldc,  \
+                                                               units, tw);                   \
+            return hipGetLastError();                                                        \
+        }                                                                                    \
         const int vpb = RowCfg<N>::VPB;                                                      \
         dim3 grid((Hc + vpb - 1) / vpb, kNumOri, ss.n);                                      \
         k_inv_rows<N><<<grid, RowCfg<N>::THREADS, vpb * fft_lds_floats<N>() * sizeof(float), s>>>( \
