@@ -195,8 +195,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         lv[l] = c->level[l];
         lp[l] = (size_t)c->levelW[l];
     }
-    // FFT groups above level 0 go to the side stream once the pyramid is on s, except while
-    // s is being captured into a graph (there every group stays on s)
+    // Work above pyramid level 0 (FFT groups and direct launches) goes to the side stream once
+    // the pyramid is on s, except while s is being captured into a graph (there it stays on s)
     hipStreamCaptureStatus capturing = hipStreamCaptureStatusActive;
     const bool useSide = c->side && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM) &&
                          hipStreamIsCapturing(s, &capturing) == hipSuccess &&
@@ -209,6 +209,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         if (st.direct) {
             const Scale& s0 = c->sc[st.index];
             const auto& g = s0.g;
+            // with a side stream a direct step holds one level class (bimp_create)
+            const hipStream_t q = useSide && g.level > 0 ? c->side : s;
             {
                 double bytes = (double)g.H * g.W;
                 int maxext = 0;
@@ -217,18 +219,18 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                     bytes += 4.0 * kNumOri * sk.Hc * sk.Wc;
                     maxext = std::max(maxext, sk.ds.ext);
                 }
-                LaunchScope ls(c, s, F_DIRECT, bytes, maxext);
+                LaunchScope ls(c, q, F_DIRECT, bytes, maxext);
                 bimpk::DirectLevels dl{};
                 for (int l = 0; l <= c->maxLevel && l < bimpk::kDirMaxLevels; ++l) {
                     dl.img[l] = lv[l];
                     dl.pitch[l] = lp[l];
                 }
                 HIPCHK(c, bimpk::launch_direct(dl, st.jobs, st.n, st.tiles,
-                                               st.lds, s0.precision == BIMP_PRECISION_FP16, s));
+                                               st.lds, s0.precision == BIMP_PRECISION_FP16, q));
             }
             if (cells) {
                 for (int k = 0; k < st.n; ++k) {
-                    const int r = launch_cells_scale(c, (size_t)(st.index + k), s);
+                    const int r = launch_cells_scale(c, (size_t)(st.index + k), q);
                     if (r) return r;
                 }
             }
@@ -526,6 +528,11 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             return bail(BIMP_E_ALLOC, "level");
     // resolution groups: consecutive FFT-form scales (ascending lambda) with one level and
     // size; a direct-form scale is a step of its own
+    // A second stream when the frame's work spans pyramid level 0 and higher levels (see
+    // run_filter); then direct launches do not mix levels either.
+    bool spans[2] = {false, false};
+    for (const auto& s : c->sc) spans[s.g.level > 0 ? 1 : 0] = true;
+    const bool sideStream = spans[0] && spans[1] && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM);
     bool prevFft = false;
     for (int i = 0; i < (int)c->sc.size(); ++i) {
         const auto& g = c->sc[i].g;
@@ -533,7 +540,8 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
             // consecutive direct scales share a launch
             if (!c->steps.empty() && c->steps.back().direct && !prevFft) {
                 Step& last = c->steps.back();
-                if (last.n < kDirectLaunchScales) {
+                const bool sameSide = (c->sc[last.index].g.level > 0) == (g.level > 0);
+                if (last.n < kDirectLaunchScales && (!sideStream || sameSide)) {
                     ++last.n;
                     continue;
                 }
@@ -562,17 +570,14 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     std::vector<int> sizes;
     const unsigned hw = std::thread::hardware_concurrency();
     const int threads = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
-    bool levels[2] = {false, false};
     for (const auto& gr : c->groups) {
         const Scale& s0 = c->sc[gr.first];
         const int k = s0.g.level > 0 ? 1 : 0;
-        levels[k] = true;
         rMax[k] = std::max(rMax[k], (size_t)gr.n * s0.g.H * (s0.g.FW / 2 + 1));
         const size_t ts = (size_t)kNumOri * s0.Hc * s0.g.FW;
         tMax[k] = std::max(tMax[k], ts * std::min(gr.n, inverse_batch(ts)));
     }
-    // a side stream only when both classes of FFT groups exist
-    const bool sideStream = levels[0] && levels[1] && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM);
+    // FFT groups above level 0 own R2/T2 only with a side stream
     if (!sideStream) {
         rMax[0] = std::max(rMax[0], rMax[1]);
         tMax[0] = std::max(tMax[0], tMax[1]);

@@ -77,22 +77,30 @@ def test_pitched_host_frame_equals_contiguous(B):
         assert np.array_equal(out[:wn].view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("w,h", [(640, 480), (1920, 1080)])
-def test_two_streams_equal_one_stream(B, w, h):
-    """Transforms above pyramid level 0 on the context's second stream give the maps and
-    keypoints of the same frame run on one stream, bit for bit: with the flag set at creation
-    (no second stream made) and switched on and off at run time, on the null stream and on a
-    created one."""
+@pytest.mark.parametrize("w,h,form", [(640, 480, "fp32"), (1920, 1080, "fp32"),
+                                      (1920, 1080, "bf16")])
+def test_two_streams_equal_one_stream(B, w, h, form):
+    """Work above pyramid level 0 on the context's second stream (FFT groups; in the
+    precision form, the level's own direct launch) gives the maps and keypoints of the same
+    frame run on one stream (one direct launch for every scale), bit for bit: with the flag
+    set at creation (no second stream made) and switched on and off at run time, on the null
+    stream and on a created one."""
     import oracle_lib as O
     from kpmatch import bits
     img = O.synth_frame(w, h, 60, 8)
-    opt = B.default_options()
-    opt.flags = B.FLAG_SINGLE_STREAM
-    one = B.CudaStuff(w, h, O.BASE_BANK, opt)
+
+    def options(flags):
+        opt = B.default_options()
+        opt.flags = flags
+        if form == "bf16":
+            opt.precision = B.PRECISION_BF16
+        return opt
+
+    one = B.CudaStuff(w, h, O.BASE_BANK, options(B.FLAG_SINGLE_STREAM))
     want, wn = one.detect(img, cap=1 << 16)
     wmaps = [(one.get_map(i, B.MAP_DOUBLE), one.get_map(i, B.MAP_SINGLE)) for i in range(len(one.ks))]
     assert wn > 0
-    two = B.CudaStuff(w, h, O.BASE_BANK)
+    two = B.CudaStuff(w, h, O.BASE_BANK, options(0))
     s = B.Stream()
     for flags, stream in [(0, None), (0, s.handle), (B.FLAG_SINGLE_STREAM, s.handle), (0, s.handle)]:
         two.set_flags(flags)
