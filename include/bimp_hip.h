@@ -191,8 +191,10 @@ typedef struct bimp_profile_entry {
     double total_ms;    /* summed HIP-event time */
     double bytes;       /* summed algorithmic bytes of those launches */
 } bimp_profile_entry;
-/* With BIMP_FLAG_PROFILE set, every launch is bracketed by HIP events on the caller's stream;
- * this returns the accumulated per-family totals (synchronises the last stream used). */
+/* With BIMP_FLAG_PROFILE set, every launch is bracketed by HIP events on the stream it runs
+ * on; this returns the accumulated per-family totals (waits for every bracketed launch).
+ * Launches on the context's two streams overlap, so set BIMP_FLAG_SINGLE_STREAM as well for
+ * durations that are each launch's own. */
 int bimp_get_profile(bimp_ctx* ctx, bimp_profile_entry* out, int cap, int* n);
 int bimp_reset_profile(bimp_ctx* ctx);
 int bimp_set_flags(bimp_ctx* ctx, int flags);
