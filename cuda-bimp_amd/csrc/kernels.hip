@@ -244,6 +244,15 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
         // output column of staging slot b: direct k0 + b; partner FW - (k0 + 1 + (CB-1-b))
         const int slot = partner ? CB - 1 - g : g;
         const int base = partner ? FW - k0 - CB : k0;
+        // a thread stores column b of the block, rows yy0 + i*YS (IC::THREADS is a multiple
+        // of CB); where a round takes 8 or more such rows, four LDS reads go out before their
+        // four stores instead of one read waiting before each store
+        static_assert(IC::THREADS % CB == 0, "whole rows of the block per pass");
+        constexpr int YS = IC::THREADS / CB, IT = (IC::BR + YS - 1) / YS;
+        constexpr bool BATCH = IT >= 8 && IT % 4 == 0 && IC::BR % YS == 0;
+        const int b = threadIdx.x % CB, yy0 = threadIdx.x / CB;
+        const int ks = partner ? k0 + CB - b : k0 + b;  // stored column
+        const bool wcol = partner ? (ks < half) : (ks <= half);
 #pragma unroll
         for (int rb = 0; rb < IC::RL; ++rb) {
             const int ylo = rb * IC::BR;
@@ -254,11 +263,25 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
             for (int m = 0; m < C::E / IC::RL; ++m)
                 lds2[slot * IC::SS + lane + C::TPV * m] = v[m * IC::RL + rb];
             __syncthreads();
-            for (int t = threadIdx.x; t < rows * CB; t += IC::THREADS) {
-                const int b = t % CB, yy = t / CB;
-                const int ks = partner ? k0 + CB - b : k0 + b;  // stored column
-                const bool w = partner ? (ks < half) : (ks <= half);
-                if (w) To[(size_t)(ylo + yy) * FW + base + b] = lds2[b * IC::SS + yy];
+            if constexpr (BATCH) {
+                const size_t rstep = (size_t)YS * FW;
+#pragma unroll 1
+                for (int i0 = 0; i0 < IT; i0 += 4) {
+                    float2 val[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) val[j] = lds2[b * IC::SS + yy0 + (i0 + j) * YS];
+                    float2* row0 = To + (size_t)(ylo + yy0 + i0 * YS) * FW + base + b;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (wcol && yy0 + (i0 + j) * YS < rows) row0[(size_t)j * rstep] = val[j];
+                }
+            } else {
+                for (int t = threadIdx.x; t < rows * CB; t += IC::THREADS) {
+                    const int bt = t % CB, yy = t / CB;
+                    const int kst = partner ? k0 + CB - bt : k0 + bt;  // stored column
+                    const bool w = partner ? (kst < half) : (kst <= half);
+                    if (w) To[(size_t)(ylo + yy) * FW + base + bt] = lds2[bt * IC::SS + yy];
+                }
             }
         }
         __syncthreads();  // the exchange region is reused by the next transform
