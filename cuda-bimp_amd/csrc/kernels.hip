@@ -132,11 +132,27 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
     const float2* R = Rbase + (size_t)sidx * rstride;
     float2* DT = ss.DT[sidx];
     const int kx0 = blockIdx.x * CC::CB;
-    for (int t = threadIdx.x; t < FH * CC::CB; t += CC::THREADS) {
-        const int b = t % CC::CB, y = t / CC::CB;
+    {
+        // a thread stages column b of the block, rows y0 + i*YS; eight loads go out before
+        // their LDS stores
+        static_assert(CC::THREADS % CC::CB == 0 && (FH * CC::CB) % CC::THREADS == 0, "layout");
+        constexpr int YS = CC::THREADS / CC::CB, IT = FH * CC::CB / CC::THREADS;
+        const int b = threadIdx.x % CC::CB, y0 = threadIdx.x / CC::CB;
         const int kx = kx0 + b;
-        const int ry = y < H ? y : (y < H + kY ? H - 1 : 0);
-        lds2[b * CC::S + y] = kx < ncols ? R[(size_t)ry * ldr + kx] : make_float2(0.f, 0.f);
+        const bool in = kx < ncols;
+#pragma unroll
+        for (int i0 = 0; i0 < IT; i0 += 8) {
+            float2 val[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int y = y0 + (i0 + j) * YS;
+                const int ry = y < H ? y : (y < H + kY ? H - 1 : 0);
+                val[j] = (i0 + j < IT && in) ? R[(size_t)ry * ldr + kx] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (i0 + j < IT) lds2[b * CC::S + y0 + (i0 + j) * YS] = val[j];
+        }
     }
     __syncthreads();
     const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
