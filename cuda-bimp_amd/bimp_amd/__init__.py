@@ -231,9 +231,10 @@ class DeviceBuffer:
         return arr
 
     def free(self):
-        if self.ptr:
+        # (at interpreter exit the module's `lib` may already be gone; the process is ending)
+        if getattr(self, "ptr", None) and lib is not None:
             lib.bimp_device_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         self.free()
@@ -249,9 +250,9 @@ class Stream:
         _check(lib.bimp_stream_sync(self.handle))
 
     def __del__(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and lib is not None:
             lib.bimp_stream_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
 
 class CudaStuff:
@@ -273,9 +274,9 @@ class CudaStuff:
             self.ks.append(si)
 
     def close(self):
-        if getattr(self, "ctx", None):
+        if getattr(self, "ctx", None) and lib is not None:
             lib.bimp_destroy(self.ctx)
-            self.ctx = None
+        self.ctx = None
 
     def __del__(self):
         self.close()
@@ -431,9 +432,9 @@ class Batch:
         return buf.value.decode()
 
     def close(self):
-        if getattr(self, "b", None):
+        if getattr(self, "b", None) and lib is not None:
             lib.bimp_batch_destroy(self.b)
-            self.b = None
+        self.b = None
 
     def __del__(self):
         self.close()
@@ -476,9 +477,9 @@ class Diff:
         return gp
 
     def close(self):
-        if getattr(self, "d", None):
+        if getattr(self, "d", None) and lib is not None:
             lib.bimp_diff_destroy(self.d)
-            self.d = None
+        self.d = None
 
     def __del__(self):
         self.close()
