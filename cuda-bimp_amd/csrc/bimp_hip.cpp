@@ -111,6 +111,7 @@ void free_all(bimp_ctx* c) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->evFork) (void)hipEventDestroy(c->evFork);
     if (c->evJoin) (void)hipEventDestroy(c->evJoin);
+    if (c->evCells) (void)hipEventDestroy(c->evCells);
     (void)hipFree(c->maxslots);
     (void)hipFree(c->counter);
     (void)hipFree(c->blockCounts);
@@ -205,7 +206,7 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         HIPCHK(c, hipEventRecord(c->evFork, s));
         HIPCHK(c, hipStreamWaitEvent(c->side, c->evFork, 0));
     }
-    for (const Step& st : c->steps) {
+    auto run_step = [&](const Step& st) -> int {
         if (st.direct) {
             const Scale& s0 = c->sc[st.index];
             const auto& g = s0.g;
@@ -234,7 +235,7 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                     if (r) return r;
                 }
             }
-            continue;
+            return BIMP_OK;
         }
         const Group& gr = c->groups[st.index];
         const auto& g = c->sc[gr.first].g;
@@ -298,11 +299,32 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                                                  twW, q));
             }
             if (cells) {
+                // level 0 with a second stream: the cells stage of these scales runs there,
+                // beside the next transforms on s
+                hipStream_t cq = q;
+                if (useSide && !upper) {
+                    HIPCHK(c, hipEventRecord(c->evCells, q));
+                    HIPCHK(c, hipStreamWaitEvent(c->side, c->evCells, 0));
+                    cq = c->side;
+                }
                 for (int k = 0; k < sub.n; ++k) {
-                    const int st = launch_cells_scale(c, (size_t)(gr.first + k0 + k), q);
+                    const int st = launch_cells_scale(c, (size_t)(gr.first + k0 + k), cq);
                     if (st) return st;
                 }
             }
+        }
+        return BIMP_OK;
+    };
+    // Second-stream steps are issued first, so that the level-0 cells handed to that stream
+    // below queue behind its own transforms rather than ahead of them.
+    auto on_side = [&](const Step& st) {
+        return (st.direct ? c->sc[st.index].g.level : c->sc[c->groups[st.index].first].g.level) > 0;
+    };
+    for (int pass = useSide ? 0 : 1; pass < 2; ++pass) {
+        for (const Step& st : c->steps) {
+            if (useSide && on_side(st) != (pass == 0)) continue;
+            const int r = run_step(st);
+            if (r) return r;
         }
     }
     if (useSide) {
@@ -643,7 +665,8 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     if (sideStream &&
         (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
          hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
+         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evCells, hipEventDisableTiming) != hipSuccess))
         return bail(BIMP_E_DEVICE, "side stream");
     if (anyDirect) {
         if (hipError_t e = bimpk::prepare_direct(); e != hipSuccess)

@@ -102,7 +102,7 @@ struct bimp_ctx {
     // FFT groups above pyramid level 0 run on a side stream (forked after the pyramid, joined
     // before the keypoint pass), with their own forward and inverse intermediates
     hipStream_t side = nullptr;
-    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr, evCells = nullptr;
     float2* R2 = nullptr;
     float2* T2 = nullptr;
     size_t tCap2 = 0;
