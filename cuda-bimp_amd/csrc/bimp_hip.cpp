@@ -230,8 +230,15 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                                                st.lds, s0.precision == BIMP_PRECISION_FP16, q));
             }
             if (cells) {
+                // level 0 with a second stream: the second half of these scales' cells stages
+                // run there, beside the first half on s
+                const int split = useSide && g.level == 0 && st.n > 1 ? (st.n + 1) / 2 : st.n;
+                if (split < st.n) {
+                    HIPCHK(c, hipEventRecord(c->evCells, q));
+                    HIPCHK(c, hipStreamWaitEvent(c->side, c->evCells, 0));
+                }
                 for (int k = 0; k < st.n; ++k) {
-                    const int r = launch_cells_scale(c, (size_t)(st.index + k), q);
+                    const int r = launch_cells_scale(c, (size_t)(st.index + k), k < split ? q : c->side);
                     if (r) return r;
                 }
             }
