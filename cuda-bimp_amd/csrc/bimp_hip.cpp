@@ -183,21 +183,9 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         src0 = c->level[0];
         pitch0 = W0;
     }
-    std::vector<const uint8_t*> lv(c->maxLevel + 1);
-    std::vector<size_t> lp(c->maxLevel + 1);
-    lv[0] = src0;
-    lp[0] = pitch0;
-    for (int l = 1; l <= c->maxLevel; ++l) {
-        LaunchScope ls(c, s, F_PYR,
-                       (double)c->levelW[l - 1] * c->levelH[l - 1] + (double)c->levelW[l] * c->levelH[l]);
-        HIPCHK(c, bimpk::launch_pyr_down(lv[l - 1], c->levelW[l - 1], c->levelH[l - 1], lp[l - 1],
-                                         c->level[l], c->levelW[l], c->levelH[l],
-                                         (size_t)c->levelW[l], s));
-        lv[l] = c->level[l];
-        lp[l] = (size_t)c->levelW[l];
-    }
-    // Work above pyramid level 0 (FFT groups and direct launches) goes to the side stream once
-    // the pyramid is on s, except while s is being captured into a graph (there it stays on s)
+    // Work above pyramid level 0 (the pyramid itself, FFT groups and direct launches) goes to
+    // the side stream, forked once the frame is on the device, except while s is being
+    // captured into a graph (there it stays on s).  Level-0 work reads the frame only.
     hipStreamCaptureStatus capturing = hipStreamCaptureStatusActive;
     const bool useSide = c->side && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM) &&
                          hipStreamIsCapturing(s, &capturing) == hipSuccess &&
@@ -205,6 +193,20 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
     if (useSide) {
         HIPCHK(c, hipEventRecord(c->evFork, s));
         HIPCHK(c, hipStreamWaitEvent(c->side, c->evFork, 0));
+    }
+    const hipStream_t pq = useSide ? c->side : s;
+    std::vector<const uint8_t*> lv(c->maxLevel + 1);
+    std::vector<size_t> lp(c->maxLevel + 1);
+    lv[0] = src0;
+    lp[0] = pitch0;
+    for (int l = 1; l <= c->maxLevel; ++l) {
+        LaunchScope ls(c, pq, F_PYR,
+                       (double)c->levelW[l - 1] * c->levelH[l - 1] + (double)c->levelW[l] * c->levelH[l]);
+        HIPCHK(c, bimpk::launch_pyr_down(lv[l - 1], c->levelW[l - 1], c->levelH[l - 1], lp[l - 1],
+                                         c->level[l], c->levelW[l], c->levelH[l],
+                                         (size_t)c->levelW[l], pq));
+        lv[l] = c->level[l];
+        lp[l] = (size_t)c->levelW[l];
     }
     auto run_step = [&](const Step& st) -> int {
         if (st.direct) {
