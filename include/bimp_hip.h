@@ -79,10 +79,12 @@ typedef struct bimp_options {
 
 #define BIMP_FLAG_PROFILE 1  /* time every kernel with HIP events (see bimp_get_profile) */
 /* Every launch on the caller's stream.  Without it, a context whose scales span pyramid level
- * 0 and higher levels runs the higher levels' work (FFT groups, or the precision form's direct
- * launch for those levels, with their cells) on a second stream of its own, forked from the
- * caller's stream after the pyramid and joined back before the keypoint pass (and not while
- * the caller's stream is being captured into a graph).  Read at bimp_create (no second stream
+ * 0 and higher levels runs part of a frame on a second stream of its own: the pyramid, the
+ * higher levels' work (FFT groups, or the precision form's direct launch for those levels,
+ * with their cells), and level-0 cells stages beside the remaining level-0 transforms.  That
+ * stream is forked from the caller's stream once the frame is on the device and joined back
+ * before the keypoint pass (and not used while the caller's stream is being captured into a
+ * graph).  Read at bimp_create (no second stream
  * is made, and the precision form keeps one direct launch for all levels) and at every frame.
  * The batching form's contexts set it. */
 #define BIMP_FLAG_SINGLE_STREAM 2
