@@ -112,3 +112,37 @@ def test_two_streams_equal_one_stream(B, w, h, form):
                                   d.view(np.uint32))
             assert np.array_equal(two.get_map(i, B.MAP_SINGLE, stream=stream).view(np.uint32),
                                   sm.view(np.uint32))
+
+
+@pytest.mark.parametrize("flags_name", ["two_streams", "single_stream"])
+def test_profile_counts_every_launch(B, flags_name):
+    """With BIMP_FLAG_PROFILE every launch of a frame is timed and counted, whether it ran on
+    the caller's stream or the context's second one: once per frame for the clearing and the
+    three NMS passes, once per scale for the cells stage and per pyramid level for the
+    pyramid, each with a finite positive duration."""
+    import math
+
+    import oracle_lib as O
+    w, h, frames = 640, 480, 3
+    img = O.synth_frame(w, h, 60, 9)
+    cs = B.CudaStuff(w, h, O.BASE_BANK)
+    extra = B.FLAG_SINGLE_STREAM if flags_name == "single_stream" else 0
+    cs.set_flags(B.FLAG_PROFILE | extra)
+    cs.reset_profile()
+    want, wn = None, None
+    for _ in range(frames):
+        got, n = cs.detect(img, cap=1 << 16)
+        if want is None:
+            want, wn = got, n
+        assert n == wn
+    prof = cs.profile()
+    levels = max(si.level for si in cs.ks)
+    count = {k: v["launches"] for k, v in prof.items()}
+    assert count["k_init"] == frames
+    assert count["k_pyr_down"] == frames * levels
+    assert count["k_cells"] == frames * len(cs.ks)
+    for k in ("k_nms_count", "k_nms_scan", "k_nms_write"):
+        assert count[k] == frames, k
+    assert any(k.startswith("k_inv_cols") for k in prof) and any(k.startswith("k_inv_rows") for k in prof)
+    for k, v in prof.items():
+        assert math.isfinite(v["total_ms"]) and v["total_ms"] > 0, k
