@@ -287,37 +287,39 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
         }
 
         // Epilogue: lane holds filter jf = lane%16 for pixels 4*kb .. 4*kb+3 of each group;
-        // the odd filter of orientation jf/2 sits in the neighbouring lane.
+        // the odd filter of orientation jf/2 sits in the neighbouring lane.  Over a pair of
+        // groups (g2, g2+1) the even lane finishes all four pixels of g2 and the odd lane those
+        // of g2+1, after swapping the other group's values, so that each lane stores a float4
+        // and an orientation's 32 pixels of the pair go out together.
+        static_assert(NCG % 2 == 0, "groups are finished in pairs");
+        const bool oddf = (jf & 1) != 0;
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             const int yl = row0 + j, y = y0 + yl;
 #pragma unroll
-            for (int g = 0; g < NCG; ++g) {
-                const int xl = 16 * g + 4 * kb, x = x0 + xl;
-                // the lane pair (even filter, odd filter) splits the four pixels: the even
-                // lane finishes pixels 0-1, the odd lane pixels 2-3, after swapping the other
-                // halves
-                const int tb2 = (jf & 1) ? 2 : 0;
-                const float2 bx = *reinterpret_cast<const float2*>(vs + yl * TX + xl + tb2);
-                const float bxs[2] = {bx.x, bx.y};
-                float c2[2];
+            for (int g2 = 0; g2 < NCG; g2 += 2) {
+                const int gm = g2 + (oddf ? 1 : 0);  // the group this lane finishes
+                const int xl = 16 * gm + 4 * kb, x = x0 + xl;
+                const float4 bx = *reinterpret_cast<const float4*>(vs + yl * TX + xl);
+                const float bxs[4] = {bx.x, bx.y, bx.z, bx.w};
+                float c4[4];
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const float mine = (jf & 1) ? acc[j][g][2 + q] : acc[j][g][q];
-                    const float send = (jf & 1) ? acc[j][g][q] : acc[j][g][2 + q];
+                for (int q = 0; q < 4; ++q) {
+                    const float mine = oddf ? acc[j][g2 + 1][q] : acc[j][g2][q];
+                    const float send = oddf ? acc[j][g2][q] : acc[j][g2 + 1][q];
                     const float other = __shfl_xor(send, 1, 64);
-                    const float e = ds.fscale * ((jf & 1) ? other : mine) - dce * bxs[q];
-                    const float od = ds.fscale * ((jf & 1) ? mine : other) - dco * bxs[q];
-                    c2[q] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
+                    const float e = ds.fscale * (oddf ? other : mine) - dce * bxs[q];
+                    const float od = ds.fscale * (oddf ? mine : other) - dco * bxs[q];
+                    c4[q] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
                 }
                 if (y < ds.Hc) {
-                    float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x + tb2;
-                    if (x + tb2 + 1 < ds.Wc) {
-                        *reinterpret_cast<float2*>(out) = make_float2(c2[0], c2[1]);
+                    float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x;
+                    if (x + 3 < ds.Wc) {
+                        *reinterpret_cast<float4*>(out) = make_float4(c4[0], c4[1], c4[2], c4[3]);
                     } else {
 #pragma unroll
-                        for (int q = 0; q < 2; ++q)
-                            if (x + tb2 + q < ds.Wc) out[q] = c2[q];
+                        for (int q = 0; q < 4; ++q)
+                            if (x + q < ds.Wc) out[q] = c4[q];
                     }
                 }
             }
