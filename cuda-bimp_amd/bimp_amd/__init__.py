@@ -80,7 +80,7 @@ EXPORTED = [
     "bimp_batch_last_error",
     "bimp_diff_create", "bimp_diff_destroy", "bimp_diff_response_size", "bimp_diff_set_params",
     "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error", "bimp_bgr_to_gray",
-    "bimp_keypoints_cpu_mode",
+    "bimp_keypoints_cpu_mode", "bimp_time_detect_frames", "bimp_get_level",
 ]
 
 
@@ -138,6 +138,8 @@ def _load():
         "bimp_stream_sync": (I, [P]),
         "bimp_time_detect": (I, [P, P, SZ, P, I, I, P, C.POINTER(C.c_float)]),
         "bimp_copy_kernel": (I, [P, P, SZ, P]),
+        "bimp_time_detect_frames": (I, [P, P, SZ, P, I, I, P, P]),
+        "bimp_get_level": (I, [P, I, P, SZ, P]),
         "bimp_synth_frame": (I, [I, I, I, C.c_uint64, P, SZ]),
         "bimp_batch_create": (I, [P, P, I, P, I, C.POINTER(Options), I, I, C.POINTER(P)]),
         "bimp_batch_destroy": (I, [P]),
@@ -358,6 +360,24 @@ class CudaStuff:
         self._chk(lib.bimp_time_detect(self.ctx, d_img.ptr, pitch, d_out.ptr, cap, iters,
                                        stream.handle if stream else None, C.byref(ms)))
         return ms.value
+
+    def time_detect_frames(self, d_img, pitch, d_out, cap, iters, stream):
+        """Per-frame elapsed milliseconds of `iters` back-to-back frames (numpy array)."""
+        ms = np.zeros(int(iters), np.float32)
+        self._chk(lib.bimp_time_detect_frames(self.ctx, d_img.ptr, pitch, d_out.ptr, cap,
+                                              int(iters), stream.handle if stream else None,
+                                              _ptr(ms)))
+        return ms
+
+    def get_level(self, level, stream=None):
+        """Pyramid level `level` (>= 1) of the last filtered frame, uint8 (h, w)."""
+        w, h = self.width, self.height
+        for _ in range(level):
+            w, h = (w + 1) // 2, (h + 1) // 2
+        out = np.zeros((h, w), np.uint8)
+        self._chk(lib.bimp_get_level(self.ctx, int(level), _ptr(out), w,
+                                     stream.handle if stream else None))
+        return out
 
 
 def bgr_to_gray(bgr):

@@ -3,14 +3,16 @@
 // A batch groups its frames by size.  Each size gets one or more contexts (the reference's
 // CudaStuff, one per frame size, cuda_bimp.cu:222-397); each context is one branch of the
 // graph and runs its frames back to back, so frames of one size share buffers and frames of
-// different sizes overlap on the device.  Small frames cannot fill the chip on their own, so
-// the tuner may give a size several contexts; it may also shrink a context's inverse-pass
-// sub-batch.  Neither choice changes any result: every frame's keypoints are those of
-// bimp_detect_u8 on a context of its size.
+// different sizes overlap on the device.  Every frame's keypoints are those of bimp_detect_u8
+// on a context of its size.
+//
+// The plan search that rebuilt the graph for trial plans (tune_plan != 0) is switched off:
+// bimp_batch_create returns BIMP_E_UNSUPPORTED for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -57,6 +59,18 @@ int berr(bimp_batch* b, int code, const std::string& msg) {
     return code;
 }
 
+// Phase markers of a batch build on stderr (set BIMP_TRACE to a non-empty value): a failure
+// inside the runtime then names the step it happened in.
+void phase(const char* what, int a = -1, int c = -1) {
+    const char* on = std::getenv("BIMP_TRACE");
+    if (!on || !*on) return;
+    if (a >= 0)
+        std::fprintf(stderr, "bimp_batch: %s %d/%d\n", what, a, c);
+    else
+        std::fprintf(stderr, "bimp_batch: %s\n", what);
+    std::fflush(stderr);
+}
+
 #define BCHK(b, expr)                                                                  \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -100,6 +114,7 @@ template <class F>
 int capture(bimp_batch* b, hipGraphExec_t* exec, hipGraph_t* keep, F body) {
     BCHK(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
     const int st = body();
+    phase("end capture");
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->cs, &g);
     if (st != BIMP_OK) {
@@ -107,6 +122,7 @@ int capture(bimp_batch* b, hipGraphExec_t* exec, hipGraph_t* keep, F body) {
         return st;
     }
     if (e != hipSuccess) return berr(b, BIMP_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    phase("instantiate");
     const hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
     if (ie != hipSuccess) {
         (void)hipGraphDestroy(g);
@@ -119,10 +135,9 @@ int capture(bimp_batch* b, hipGraphExec_t* exec, hipGraph_t* keep, F body) {
     return BIMP_OK;
 }
 
-// The batch graph: fork on the capture stream, one branch per context, join.  With
-// `per_frame`, also one graph per frame (the latency of a frame alone); the tuner's trial
-// plans skip those.
-int build(bimp_batch* b, bool per_frame = true) {
+// The batch graph: fork on the capture stream, one branch per context, join; then one graph
+// per frame (the latency of a frame alone).
+int build(bimp_batch* b) {
     drop_graphs(b);
     std::vector<std::pair<bimp_ctx*, std::vector<int>>> branches;
     for (auto& c : b->cls) {
@@ -146,6 +161,7 @@ int build(bimp_batch* b, bool per_frame = true) {
         BCHK(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->ev.push_back(e);
     }
+    phase("capture batch graph, branches", (int)branches.size(), (int)branches.size());
     int st = capture(b, &b->exec, &b->graph, [&]() -> int {
         BCHK(b, hipEventRecord(b->ev[0], b->cs));
         for (size_t j = 0; j < branches.size(); ++j) {
@@ -160,9 +176,10 @@ int build(bimp_batch* b, bool per_frame = true) {
         }
         return BIMP_OK;
     });
-    if (st || !per_frame) return st;
+    if (st) return st;
     b->frameExec.assign(b->nframes, nullptr);
     for (int f = 0; f < b->nframes; ++f) {
+        phase("capture frame graph", f, b->nframes);
         bimp_ctx* x = b->cls[b->frameClass[f]].ctx[0];
         st = capture(b, &b->frameExec[f], nullptr, [&]() { return enqueue_frame(b, x, f, b->cs); });
         if (st) return st;
@@ -187,52 +204,6 @@ int time_batch(bimp_batch* b, int iters, hipStream_t s, float* ms) {
     return BIMP_OK;
 }
 
-// Greedy plan search: for each frame size (smallest first) add contexts while the batch gets
-// faster by more than 2%, then try the smallest inverse-pass sub-batch.  Contexts a trial
-// rejects are released only once the final graphs exist, so no instantiated graph ever refers
-// to freed memory.
-int tune(bimp_batch* b, hipStream_t s) {
-    std::vector<bimp_ctx*> rejected;
-    const int st = [&]() -> int {
-        float best = 0;
-        int r = build(b, false);
-        if (r) return r;
-        if ((r = time_batch(b, 3, s, &best))) return r;
-        std::vector<size_t> order(b->cls.size());
-        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-        std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-            return (long)b->cls[x].w * b->cls[x].h < (long)b->cls[y].w * b->cls[y].h;
-        });
-        for (size_t ci : order) {
-            SizeClass& c = b->cls[ci];
-            const size_t maxK = std::min<size_t>(4, c.frames.size());
-            while (c.ctx.size() < maxK) {
-                if ((r = ensure_ctx(b, c, c.ctx.size()))) return r;
-                float t = 0;
-                if ((r = build(b, false)) || (r = time_batch(b, 3, s, &t))) return r;
-                if (t < 0.98f * best) {
-                    best = t;
-                    continue;
-                }
-                rejected.push_back(c.ctx.back());
-                c.ctx.pop_back();
-                break;
-            }
-            c.invBatch = 1;
-            float t = 0;
-            if ((r = build(b, false)) || (r = time_batch(b, 3, s, &t))) return r;
-            if (t < 0.98f * best)
-                best = t;
-            else
-                c.invBatch = 0;
-        }
-        return build(b, true);
-    }();
-    (void)hipStreamSynchronize(s);
-    for (auto* x : rejected) bimp_destroy(x);
-    return st;
-}
-
 }  // namespace
 
 extern "C" {
@@ -242,6 +213,9 @@ int bimp_batch_create(const int* widths, const int* heights, int nframes, const 
     if (!out) return BIMP_E_ARG;
     *out = nullptr;
     if (!widths || !heights || nframes <= 0 || !lambdas || n <= 0 || cap < 0) return BIMP_E_ARG;
+    // The plan search (trial graphs rebuilt with more contexts per size) ended twice in a host
+    // segmentation fault inside this call, cause not found: it stays switched off.
+    if (tune_plan) return BIMP_E_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BIMP_E_DEVICE;
     bimp_batch* b = new bimp_batch;
@@ -288,6 +262,7 @@ int bimp_batch_create(const int* widths, const int* heights, int nframes, const 
         hipMemset(b->counts, 0, (size_t)nframes * sizeof(int)) != hipSuccess ||
         hipStreamCreate(&b->cs) != hipSuccess)
         return bail(BIMP_E_ALLOC);
+    phase("create contexts", (int)b->cls.size(), (int)b->cls.size());
     for (auto& c : b->cls) {
         const int st = ensure_ctx(b, c, 0);
         if (st) {
@@ -295,12 +270,14 @@ int bimp_batch_create(const int* widths, const int* heights, int nframes, const 
             return bail(st);
         }
     }
-    const int st = tune_plan ? tune(b, b->cs) : build(b);
+    phase("build graphs");
+    const int st = build(b);
     if (st) {
         std::fprintf(stderr, "bimp_batch_create: %s\n", b->err.c_str());
         return bail(st);
     }
     (void)hipStreamSynchronize(b->cs);
+    phase("created");
     *out = b;
     return BIMP_OK;
 }

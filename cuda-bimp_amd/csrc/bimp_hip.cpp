@@ -1015,6 +1015,53 @@ int bimp_time_detect(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypo
     return BIMP_OK;
 }
 
+int bimp_time_detect_frames(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypoint* d_out,
+                            int cap, int iters, bimp_stream_t stream, float* frame_ms) {
+    if (!c || !d_img || !d_out || iters <= 0 || !frame_ms) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<hipEvent_t> ev((size_t)iters + 1, nullptr);
+    auto release = [&]() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    for (auto& e : ev) {
+        const hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) {
+            release();
+            return hip_fail(c, r, "hipEventCreate");
+        }
+    }
+    hipError_t r = hipEventRecord(ev[0], s);
+    for (int it = 0; it < iters && r == hipSuccess; ++it) {
+        int st = run_filter(c, d_img, pitch, 1, true, s);
+        if (!st) st = run_keypoints(c, d_out, cap, 0, s);
+        if (st) {
+            (void)hipStreamSynchronize(s);
+            release();
+            return st;
+        }
+        r = hipEventRecord(ev[(size_t)it + 1], s);
+    }
+    if (r == hipSuccess) r = hipEventSynchronize(ev[(size_t)iters]);
+    for (int it = 0; it < iters && r == hipSuccess; ++it)
+        r = hipEventElapsedTime(&frame_ms[it], ev[(size_t)it], ev[(size_t)it + 1]);
+    release();
+    if (r != hipSuccess) return hip_fail(c, r, "per-frame events");
+    if (c->opt.flags & BIMP_FLAG_PROFILE) collect_profile(c);
+    return BIMP_OK;
+}
+
+int bimp_get_level(bimp_ctx* c, int level, uint8_t* dst, size_t pitch, bimp_stream_t stream) {
+    if (!c || !dst || level < 1 || level > c->maxLevel) return BIMP_E_ARG;
+    const int w = c->levelW[level], h = c->levelH[level];
+    if (pitch < (size_t)w) return BIMP_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(c, hipMemcpy2DAsync(dst, pitch, c->level[level], (size_t)w, (size_t)w, (size_t)h,
+                               hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return BIMP_OK;
+}
+
 // CPU keypoint mode, cuda_keypoints (cuda_bimp.cu:745-808): the reference's getKeypoints with
 // cpu_mode (cuda_bimp.cu:117-124) downloads D and S and extracts keypoints on the host with
 // its own rules: 8-neighbour dilation ignoring outside neighbours, threshold 0.104*max(D) for

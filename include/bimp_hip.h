@@ -215,6 +215,16 @@ int bimp_stream_sync(bimp_stream_t s);
  * events on `stream`; writes the elapsed milliseconds. */
 int bimp_time_detect(bimp_ctx* ctx, const uint8_t* d_img, size_t pitch, bimp_keypoint* d_out,
                      int cap, int iters, bimp_stream_t stream, float* elapsed_ms);
+/* As bimp_time_detect, with one event pair per frame: frame_ms[k] is frame k's elapsed
+ * milliseconds (iters entries; frames run back to back, so their sum is the whole run). */
+int bimp_time_detect_frames(bimp_ctx* ctx, const uint8_t* d_img, size_t pitch,
+                            bimp_keypoint* d_out, int cap, int iters, bimp_stream_t stream,
+                            float* frame_ms);
+/* Pyramid level `level` (1 .. deepest level of the bank) of the last filtered frame, copied to
+ * host memory (uint8, (w+1)/2 x (h+1)/2 per level, row pitch `pitch`); synchronises the stream.
+ * The reference computes it with cv::cuda::pyrDown (cuda_bimp.cu:473-479) and does not expose
+ * it. */
+int bimp_get_level(bimp_ctx* ctx, int level, uint8_t* dst, size_t pitch, bimp_stream_t stream);
 /* A plain device copy kernel of known size, used to calibrate HBM byte counters. */
 int bimp_copy_kernel(void* dst, const void* src, size_t bytes, bimp_stream_t stream);
 
@@ -246,8 +256,9 @@ int bimp_bgr_to_gray(const uint8_t* bgr, size_t bgr_pitch, int width, int height
  * The batch owns one device input slot per frame (fill it, then run), per-frame keypoint
  * outputs (cap records each, (scale, row, column) order) and per-frame counts.  The whole
  * batch is one hipGraph: one branch per context, frames of one size spread over that size's
- * contexts.  With tune != 0, creation times the candidate plans on the device (contexts per
- * frame size, inverse-pass sub-batch per context) and keeps the fastest. */
+ * contexts.  One context per frame size; tune != 0 (a plan search over contexts per size)
+ * is switched off and returns BIMP_E_UNSUPPORTED.  With BIMP_TRACE set in the environment,
+ * creation prints its steps to stderr. */
 typedef struct bimp_batch bimp_batch;
 
 int bimp_batch_create(const int* widths, const int* heights, int nframes, const float* lambdas,

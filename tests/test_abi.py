@@ -90,6 +90,11 @@ def test_errors_are_status_codes():
     assert B.lib.bimp_batch_create(B._ptr(w), B._ptr(w), 0, B._ptr(lam), 1, None, 10, 0,
                                    C.byref(b)) == B.E_ARG
     assert B.lib.bimp_batch_destroy(None) == B.E_ARG
+    # the plan search is switched off: asking for it is refused before any device work
+    w2 = np.array([64, 32], np.int32)
+    assert B.lib.bimp_batch_create(B._ptr(w2), B._ptr(w2), 2, B._ptr(lam), 1, None, 10, 1,
+                                   C.byref(b)) == B.E_UNSUPPORTED
+    assert not b.value
 
 
 def test_facade_header_compiles(tmp_path):

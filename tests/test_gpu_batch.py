@@ -1,11 +1,15 @@
 """GPU tests of the batching form (bimp_batch_*): one hipGraph launch over frames of uneven
-sizes gives every frame exactly the keypoints bimp_detect_u8 gives it alone, whatever plan
-the tuner picks, and replays are identical."""
+sizes gives every frame exactly the keypoints bimp_detect_u8 gives it alone, and replays are
+identical; the 64-frame workload of five sizes, with one frame of each size against the FP64
+oracle."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import oracle_lib as O
-from kpmatch import bits
+from kpmatch import agreement, bits, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -32,11 +36,13 @@ def _single(B, frames, options=None):
     return ref
 
 
-# The tuned case ended once in a host segmentation fault inside bimp_batch_create; its host
-# logic passes AddressSanitizer over every tuner branch with a stand-in runtime, and the cause
-# is not found, so the case stays off the GPU until one is.
-TUNED_SKIP = pytest.mark.skip(reason="host segfault in tuned bimp_batch_create seen once on the "
-                                     "GPU, cause not found")
+# The tuned case ended in a host segmentation fault inside bimp_batch_create, and an untuned
+# build with a forked second stream inside the capture did once too; the cause is not found.
+# The plan search is switched off (bimp_batch_create returns BIMP_E_UNSUPPORTED for it) and
+# batch contexts run on one stream, so the case stays off the GPU.
+TUNED_SKIP = pytest.mark.skip(reason="host segfaults in bimp_batch_create (tuned plan search; "
+                                     "untuned with a second stream in the capture), cause not "
+                                     "found: the plan search is switched off")
 
 
 @pytest.mark.parametrize("tune", [False, pytest.param(True, marks=TUNED_SKIP)])
@@ -105,4 +111,50 @@ def test_batch_overflow_reports_total(B):
     got, n = bt.results(0)
     assert n == ref[0][1] and len(got) == min(5, n)
     assert np.array_equal(bits(got), bits(ref[0][0][:len(got)]))
+    bt.close()
+
+
+WORKLOAD_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def test_batch_workload_64_frames(B, capfd):
+    """64 frames, sizes drawn uniformly (seeded) from five sizes, 40 shapes each, frame i from
+    seed 1000 + i: every frame's keypoints are bit-identical to bimp_detect_u8 on a context of
+    its size; for the first frame of each size the response maps (of that single-frame run,
+    whose keypoints the batch reproduces bit for bit) are within 1e-4 of the FP64 oracle and
+    its keypoints agree >= 99% with the oracle's."""
+    rng = np.random.default_rng(3)
+    sizes = [WORKLOAD_SIZES[k] for k in rng.integers(0, len(WORKLOAD_SIZES), 64)]
+    frames = [B.synth_frame(w, h, 40, 1000 + i) for i, (w, h) in enumerate(sizes)]
+    os.environ["BIMP_TRACE"] = "1"
+    with capfd.disabled():
+        print(f"\nbatch of {len(sizes)} frames, sizes {sorted(set(sizes))}", flush=True)
+        bt = B.Batch(sizes, O.BASE_BANK, cap=1 << 15)
+    print("plan:", bt.plan(), flush=True)
+    for i, img in enumerate(frames):
+        bt.upload(i, img)
+    s = B.Stream()
+    bt.run(s)
+    s.sync()
+    O.lib().bo_set_threads(min(16, os.cpu_count() or 1))
+    checked = set()
+    ctx = {}
+    for i, ((w, h), img) in enumerate(zip(sizes, frames)):
+        if (w, h) not in ctx:
+            ctx[(w, h)] = B.CudaStuff(w, h, O.BASE_BANK)
+        cs = ctx[(w, h)]
+        want, wn = cs.detect(img, cap=1 << 15)
+        got, n = bt.results(i)
+        assert n == wn and np.array_equal(bits(got), bits(want)), (i, w, h, n, wn)
+        if (w, h) in checked:
+            continue
+        checked.add((w, h))
+        orc = O.Oracle(w, h, O.BASE_BANK)
+        kps, m, Ds, _ = orc.detect(img, cap=1 << 15, want_maps=True)
+        err = max(rel_err(cs.get_map(k, B.MAP_DOUBLE), Ds[k]) for k in range(len(cs.ks)))
+        agr = agreement(got, kps)
+        print(f"frame {i} {w}x{h}: keypoints {n} oracle {m} agreement {agr:.4f} "
+              f"max D rel err {err:.2e}", file=sys.stderr, flush=True)
+        assert err < 1e-4 and agr >= 0.99, (i, w, h)
+    assert checked == set(sizes)
     bt.close()

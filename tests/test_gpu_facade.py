@@ -13,7 +13,10 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_context_facade_matches_c_abi(tmp_path):
+@pytest.mark.parametrize("bank", ["base", "default"])
+def test_context_facade_matches_c_abi(tmp_path, bank):
+    """bank "default": the Context's own makeLambdasLog(4, 64, 2) (9 scales, pyramid level 4,
+    the 7.9999995 wavelength); "base": the 8-wavelength bank 4 * 2^(k/4)."""
     import bimp_amd as B
     if B.device_count() < 1:
         pytest.fail("no GPU visible to the HIP library")
@@ -27,7 +30,8 @@ def test_context_facade_matches_c_abi(tmp_path):
     frame = tmp_path / "frame.u8"
     img.tofile(frame)
     out = tmp_path / "out.bin"
-    subprocess.check_call([str(exe), str(w), str(h), str(frame), str(out)], timeout=120)
+    args = [str(exe), str(w), str(h), str(frame), str(out)] + (["default"] if bank == "default" else [])
+    subprocess.check_call(args, timeout=120)
     raw = np.fromfile(out, np.uint8)
     n = int(raw[:4].view(np.int32)[0])
     kps = raw[4:4 + 28 * n].view(B.KEYPOINT_DTYPE)
@@ -36,7 +40,9 @@ def test_context_facade_matches_c_abi(tmp_path):
     o += 28 * n
     nc = int(raw[o:o + 4].view(np.int32)[0])
     cpu = raw[o + 4:o + 4 + 28 * nc].view(B.KEYPOINT_DTYPE)
-    cs = B.CudaStuff(w, h, O.BASE_BANK)
+    lam = B.make_lambdas_log(4, 64, 2) if bank == "default" else O.BASE_BANK
+    cs = B.CudaStuff(w, h, lam)
+    assert len(cs.ks) == (9 if bank == "default" else 8)
     want, m = cs.detect(img)
     assert n == m
     assert np.array_equal(bits(kps), bits(want))
