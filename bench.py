@@ -9,7 +9,7 @@ Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel, CPU 
 The same line carries the precision form (BF16 / FP16 direct convolution on the matrix cores
 for every scale): its frames/s, speed-up over the plain
 path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle,
-the batching form (64 frames of five sizes as one tuned hipGraph launch): aggregate
+the batching form (64 frames of five sizes as one hipGraph launch): aggregate
 megapixels/s and per-frame latency, the headless video loop (host frames in, keypoints out,
 frames/s), and the differentiation form (16 FP32 512x512 images
 through the libtorch autograd Function): backward time per image and gradient checks.
@@ -44,6 +44,40 @@ def b_alg(cs):
     return tot
 
 
+# The stage-boundary model's bytes per scale (A = H*W of the scale, C = FH*(FW/2+1)), split
+# by the kernel that carries each stage: forward rows read the image and write the row
+# intermediate; forward columns read it and write the spectrum D; the inverse column pass
+# reads D once and the bank (real Ke + imaginary Ko spectra, 8 orientations) and writes the 16
+# inverse half-spectra; the inverse row pass reads those and writes the 8 magnitudes; the
+# cells stage reads the magnitudes and writes D and S; keypoints read D twice.
+STAGE_BYTES = {
+    "k_fwd_rows": lambda A, C: 4 * A + 8 * C,
+    "k_fwd_cols": lambda A, C: 8 * C + 8 * C,
+    "k_inv_cols": lambda A, C: 8 * C + 64 * C + 128 * C,
+    "k_inv_rows": lambda A, C: 128 * C + 32 * A,
+    "k_inv_rows_p": lambda A, C: 128 * C + 32 * A,
+    "k_cells": lambda A, C: 32 * A + 8 * A,
+}
+
+
+def alg_bytes_per_launch(cs, kernel, launches_per_frame):
+    """Algorithmic bytes of one launch of `kernel` ("family<N>"): the stage's bytes summed over
+    the scales whose transform length matches N (FH for column passes, FW for row passes),
+    divided by the launches per frame."""
+    fam, _, size = kernel.partition("<")
+    size = int(size.rstrip(">")) if size else 0
+    f = STAGE_BYTES.get(fam)
+    if f is None:
+        return 0.0
+    tot = 0.0
+    for si in cs.ks:
+        n = si.fft_h if "cols" in fam else si.fft_w
+        if size and n != size:
+            continue
+        tot += f(si.data_h * si.data_w, si.fft_h * (si.fft_w // 2 + 1))
+    return tot / max(launches_per_frame, 1e-9)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -55,22 +89,57 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(img, frames):
-    """The CPU restatement (FP32 FFT path, OpenMP over rows/columns/pixels) on the host."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
+def _cpu_run(O, w, h, nshapes, seed, threads, frames):
+    """Frames/s of the FP32 CPU restatement on one frame size, after one warm-up frame."""
+    img = O.synth_frame(w, h, nshapes, seed)
     O.lib().bo_set_threads(threads)
-    orc = O.Oracle(W, H, bank(), fp64=False)
-    orc.detect(img)  # first frame warms caches and page mappings
+    orc = O.Oracle(w, h, bank(), fp64=False)
+    orc.detect(img, cap=1 << 16)  # warms caches and page mappings
     t0 = time.perf_counter()
     for _ in range(frames):
-        _, n = orc.detect(img)
+        _, n = orc.detect(img, cap=1 << 16)
     dt = time.perf_counter() - t0
-    return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+    return {"frame": [w, h], "threads": threads, "frames": frames,
+            "fps": round(frames / dt, 4), "s_per_frame": round(dt / frames, 4), "keypoints": n}
+
+
+def _cpu_nms_stage(O, w, h, nshapes, seed, frames=5):
+    """The reference's host keypoint pass (cuda_keypoints, cuda_bimp.cu:745-860: the only stage
+    its CPU mode runs on the host), timed alone on one thread over the frame's D and S maps."""
+    img = O.synth_frame(w, h, nshapes, seed)
+    O.lib().bo_set_threads(min(16, os.cpu_count() or 1))
+    orc = O.Oracle(w, h, bank(), fp64=False)
+    _, _, Ds, Ss = orc.detect(img, cap=1 << 16, want_maps=True)
+    O.lib().bo_set_threads(1)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        n = 0
+        for i in range(len(orc.scales)):
+            n += orc.nms_cpu_mode(i, Ds[i], Ss[i], cap=1 << 16)[1]
+    dt = time.perf_counter() - t0
+    return {"frame": [w, h], "threads": 1, "frames": frames,
+            "ms_per_frame": round(dt / frames * 1e3, 3), "keypoints": n}
+
+
+def cpu_baseline(img, frames):
+    """The CPU restatement (FP32 FFT path, OpenMP over rows/columns/pixels) on the host: the
+    1920x1080 frame with every host thread (the reported value) and with one, the 640x480
+    frame likewise, and the host keypoint pass alone."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    nproc = os.cpu_count() or 1
+    allc = min(16, nproc)
+    runs = [_cpu_run(O, W, H, NSHAPES, SEED, allc, frames),
+            _cpu_run(O, W, H, NSHAPES, SEED, 1, 1),
+            _cpu_run(O, 640, 480, 40, 0, allc, 10),
+            _cpu_run(O, 640, 480, 40, 0, 1, 10)]
+    nms = [_cpu_nms_stage(O, 640, 480, 40, 0), _cpu_nms_stage(O, W, H, NSHAPES, SEED, 2)]
+    main = runs[0]
+    return {"value": main["fps"], "unit": "frames/s", "cores": allc, "kind": "port",
             "sample": f"{frames} frames of 1920x1080 after one warm-up frame, FP32 FFT "
-                      f"restatement, {threads} OpenMP threads on {cpu_model()} "
-                      f"(nproc {os.cpu_count()}), {n} keypoints/frame"}
+                      f"restatement, {allc} OpenMP threads on {cpu_model()} "
+                      f"(nproc {nproc}), {main['keypoints']} keypoints/frame",
+            "cpu_model": cpu_model(), "nproc": nproc, "runs": runs, "host_keypoint_pass": nms}
 
 
 MFMA_PEAK_TFLOPS = 2500.0  # dense BF16 / FP16 matrix peak of MI355X
@@ -166,12 +235,13 @@ BATCH_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
 
 def batch_form(B, stream, frames=64, seed=3, iters=3):
     """64 frames with sizes drawn uniformly from BATCH_SIZES (seeded), 40 shapes each, frame i
-    from seed 1000 + i; one tuned hipGraph launch per batch."""
+    from seed 1000 + i; one hipGraph launch per batch, one context per frame size (the plan
+    search is switched off in the library)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sizes = [BATCH_SIZES[k] for k in rng.integers(0, len(BATCH_SIZES), frames)]
     t0 = time.perf_counter()
-    bt = B.Batch(sizes, bank(), tune=True)
+    bt = B.Batch(sizes, bank())
     setup = time.perf_counter() - t0
     for i, (w, h) in enumerate(sizes):
         bt.upload(i, B.synth_frame(w, h, 40, 1000 + i))
@@ -388,9 +458,10 @@ def main():
         cs.time_detect(d_img, W, d_out, cap, args.warmup, stream)
     stream.sync()
     t0 = time.perf_counter()
-    ms = cs.time_detect(d_img, W, d_out, cap, args.steps, stream)
+    per_frame = cs.time_detect_frames(d_img, W, d_out, cap, args.steps, stream)
     stream.sync()
     wall = time.perf_counter() - t0
+    ms = float(per_frame.astype(np.float64).sum())
     ms_per_step = ms / args.steps
     fps = 1000.0 / ms_per_step
     # every keypoint of the frame (the reference's 8000 cap would truncate the comparison)
@@ -407,7 +478,8 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dom]
     avg_ms = d["total_ms"] / d["launches"]
-    bytes_per_launch = d["bytes"] / d["launches"]
+    layout_bytes_per_launch = d["bytes"] / d["launches"]
+    bytes_per_launch = alg_bytes_per_launch(cs, dom, d["launches"] / args.profile_frames)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     frame_kernel_ms = sum(v["total_ms"] for v in kern.values()) / args.profile_frames
     balg = b_alg(cs)
@@ -436,7 +508,15 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(dom),
                      "avg_launch_us": round(avg_ms * 1e3, 2),
-                     "alg_bytes_per_launch": int(bytes_per_launch)},
+                     "alg_bytes_per_launch": int(bytes_per_launch),
+                     "alg_bytes_model": "stage-boundary model per scale (W*H + sum 84A + 352C "
+                                        "per frame), this stage's share",
+                     "layout_bytes_per_launch": int(layout_bytes_per_launch)},
+        "frame_ms": {"median": round(float(np.median(per_frame)), 4),
+                     "p99": round(float(np.percentile(per_frame, 99)), 4),
+                     "min": round(float(per_frame.min()), 4),
+                     "max": round(float(per_frame.max()), 4),
+                     "timing": "one HIP event pair per frame, frames back to back"},
         "frame_b_alg": {"bytes": int(balg), "gbs": round(balg / (ms_per_step * 1e-3) / 1e9, 1),
                         "frac": round(balg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels_ms_per_frame": {k: round(v["total_ms"] / args.profile_frames, 4)
@@ -455,12 +535,14 @@ def main():
         out["keypoint_agreement_vs_fp64_oracle"] = round(agreement(kps, ref_kps), 4)
         forms = {}
         # every scale direct (the library's default) at full kW, 8 sigma and 6 sigma, and the
-        # split the workload names: lambda <= 8 direct at 6 sigma, the rest FFT-FP32
+        # split the workload names (lambda <= 8 direct, the rest FFT-FP32) at 8 and 6 sigma
         for name, prec, nsig, maxl in (("bf16", B.PRECISION_BF16, 0.0, 64.0),
                                        ("fp16", B.PRECISION_FP16, 0.0, 64.0),
                                        ("bf16_8sigma", B.PRECISION_BF16, 8.0, 64.0),
                                        ("fp16_8sigma", B.PRECISION_FP16, 8.0, 64.0),
                                        ("bf16_6sigma", B.PRECISION_BF16, 6.0, 64.0),
+                                       ("bf16_8sigma_lambda8", B.PRECISION_BF16, 8.0, 8.0),
+                                       ("fp16_8sigma_lambda8", B.PRECISION_FP16, 8.0, 8.0),
                                        ("bf16_6sigma_lambda8", B.PRECISION_BF16, 6.0, 8.0),
                                        ("fp16_6sigma_lambda8", B.PRECISION_FP16, 6.0, 8.0)):
             forms[name] = precision_form(B, name, prec, nsig, ms_per_step, ref_kps, ref_maps,
