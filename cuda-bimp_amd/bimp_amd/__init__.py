@@ -81,6 +81,7 @@ EXPORTED = [
     "bimp_diff_create", "bimp_diff_destroy", "bimp_diff_response_size", "bimp_diff_set_params",
     "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error", "bimp_bgr_to_gray",
     "bimp_keypoints_cpu_mode", "bimp_time_detect_frames", "bimp_get_level",
+    "bimp_color_features", "bimp_match_knn2", "bimp_ratio_matches",
 ]
 
 
@@ -140,6 +141,9 @@ def _load():
         "bimp_copy_kernel": (I, [P, P, SZ, P]),
         "bimp_time_detect_frames": (I, [P, P, SZ, P, I, I, P, P]),
         "bimp_get_level": (I, [P, I, P, SZ, P]),
+        "bimp_color_features": (I, [P, I, I, P, SZ, I, I, P, I, P]),
+        "bimp_match_knn2": (I, [P, I, I, P, I, I, I, P, P, P]),
+        "bimp_ratio_matches": (I, [P, P, I, C.c_double, P, I, C.POINTER(I)]),
         "bimp_synth_frame": (I, [I, I, I, C.c_uint64, P, SZ]),
         "bimp_batch_create": (I, [P, P, I, P, I, C.POINTER(Options), I, I, C.POINTER(P)]),
         "bimp_batch_destroy": (I, [P]),
@@ -391,6 +395,66 @@ def bgr_to_gray(bgr):
     d.download(out)
     d.free()
     return out
+
+
+COLOR_DIMENSIONS = 24
+MATCH_DTYPE = np.dtype([("query_idx", "<i4"), ("train_idx", "<i4"), ("img_idx", "<i4"),
+                        ("distance", "<f4")])
+
+
+def color_features(kp_matrix, bgr):
+    """Colour histogram (n x 24 floats) of the keypoints of a 7-row SURF_CUDA matrix over a
+    BGR frame (H, W, 3), computed on the device (addColorFeatures, cuda_bimp.cuh:8-71)."""
+    mat = np.ascontiguousarray(kp_matrix, np.float32)
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    h, w, c = bgr.shape
+    assert c == 3 and mat.shape[0] == 7
+    n = mat.shape[1]
+    out = np.zeros((n, COLOR_DIMENSIONS), np.float32)
+    if n == 0:
+        return out
+    dk, db, dd = DeviceBuffer(mat.nbytes), DeviceBuffer(bgr.nbytes), DeviceBuffer(out.nbytes)
+    dk.upload(mat)
+    db.upload(bgr)
+    _check(lib.bimp_color_features(dk.ptr, n, n, db.ptr, 3 * w, w, h, dd.ptr, COLOR_DIMENSIONS,
+                                   None))
+    dd.download(out)
+    return out
+
+
+def match_knn2(query, train):
+    """Two nearest train rows (L2) of every query row on the device: (idx2, dist2), each
+    (nq, 2) (cv::cuda BFMatcher knnMatch, k = 2)."""
+    q = np.ascontiguousarray(query, np.float32)
+    t = np.ascontiguousarray(train, np.float32)
+    nq, dim = q.shape
+    nt = t.shape[0]
+    assert t.shape[1] == dim
+    idx = np.zeros((nq, 2), np.int32)
+    dist = np.zeros((nq, 2), np.float32)
+    if nq == 0:
+        return idx, dist
+    dq, dt = DeviceBuffer(q.nbytes), DeviceBuffer(max(t.nbytes, 4))
+    di, dd = DeviceBuffer(idx.nbytes), DeviceBuffer(dist.nbytes)
+    dq.upload(q)
+    if nt:
+        dt.upload(t)
+    _check(lib.bimp_match_knn2(dq.ptr, nq, dim, dt.ptr, nt, dim, dim, di.ptr, dd.ptr, None))
+    di.download(idx)
+    dd.download(dist)
+    return idx, dist
+
+
+def ratio_matches(idx2, dist2, ratio=0.6):
+    """Matches passing the ratio test (downloadMatches, bimp.cpp:208-240), cv::DMatch fields."""
+    idx2 = np.ascontiguousarray(idx2, np.int32)
+    dist2 = np.ascontiguousarray(dist2, np.float32)
+    nq = idx2.shape[0]
+    out = np.zeros(max(nq, 1), MATCH_DTYPE)
+    n = C.c_int(0)
+    _check(lib.bimp_ratio_matches(_ptr(idx2), _ptr(dist2), nq, float(ratio), _ptr(out), nq,
+                                  C.byref(n)))
+    return out[:n.value].copy()
 
 
 class Batch:

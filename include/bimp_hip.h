@@ -249,6 +249,45 @@ int bimp_keypoints_cpu_mode(bimp_ctx* ctx, bimp_keypoint* out, int cap, int* cou
 int bimp_bgr_to_gray(const uint8_t* bgr, size_t bgr_pitch, int width, int height, int bgr_on_device,
                      uint8_t* d_gray, size_t gray_pitch, bimp_stream_t stream);
 
+/* ---- Descriptors and matching (the keypoints' consumers) --------------------------------
+ * Context::getDescriptors (bimp.cpp:32-59) appends BIMP_COLOR_DIMENSIONS colour-histogram
+ * floats to each keypoint's descriptor (columns SURF_DIMENSIONS..DIMENSIONS of a
+ * DIMENSIONS-wide row, cuda_bimp.hpp:17-19). */
+#define BIMP_SURF_DIMENSIONS 128
+#define BIMP_COLOR_DIMENSIONS 24
+#define BIMP_DIMENSIONS (BIMP_SURF_DIMENSIONS + BIMP_COLOR_DIMENSIONS)
+
+/* cv::DMatch field order (queryIdx, trainIdx, imgIdx, distance). */
+typedef struct {
+    int query_idx, train_idx, img_idx;
+    float distance;
+} bimp_match;
+
+/* addColorFeatures (cuda_bimp.cu:952-958, kernel cuda_bimp.cuh:8-71): for the n keypoints of
+ * a device 7-row SURF_CUDA matrix (leading dimension ld), 24 floats per keypoint at
+ * d_desc + i*desc_ld: 8 bins of B, G, R over the pixels of a (2p) x (2p) window at
+ * ((int)x - p, (int)y - p), p = (int)(log2(size)/2), of a device BGR frame (3 bytes per pixel,
+ * row pitch bgr_pitch).  The reference's arithmetic: a byte is read as a signed char v and
+ * counts in bin k when 31.875k < v < 31.875(k+1) (bytes 0 and >= 128 count nowhere), and bins
+ * are scaled by 0.5 / (3 x pixels inside the image).  Window pixels outside the frame are
+ * skipped (the reference's inclusive bounds read one column and one row past it).  No
+ * synchronisation. */
+int bimp_color_features(const float* d_kp, int ld, int n, const uint8_t* d_bgr, size_t bgr_pitch,
+                        int width, int height, float* d_desc, int desc_ld, bimp_stream_t stream);
+/* Brute-force 2-nearest-neighbour matching by L2 distance (cv::cuda::DescriptorMatcher
+ * createBFMatcher + knnMatchAsync with k = 2, bimp.cpp:202-204): for each of nq device query
+ * rows (q_ld floats apart) the two train rows (t_ld apart) with the smallest distances,
+ * sqrt of the FP32 sum of squared differences over dim dimensions in order; ties go to the
+ * lower train index.  d_idx2 / d_dist2 get 2 entries per query (index -1 and +inf when there
+ * is no second row).  No synchronisation. */
+int bimp_match_knn2(const float* d_query, int nq, int q_ld, const float* d_train, int nt,
+                    int t_ld, int dim, int* d_idx2, float* d_dist2, bimp_stream_t stream);
+/* downloadMatches (bimp.cpp:208-240), host memory: query i keeps (i, idx2[2i], 0, dist2[2i])
+ * when dist2[2i] < ratio * dist2[2i+1] (the reference's ratio is 0.6, compared in double).
+ * Returns BIMP_W_OVERFLOW with *count the total when more than cap matches pass. */
+int bimp_ratio_matches(const int* idx2, const float* dist2, int nq, double ratio,
+                       bimp_match* out, int cap, int* count);
+
 /* ---- Batching form --------------------------------------------------------------------
  * Many frames of uneven sizes in one launch.  The reference has no batch entry: its caller
  * runs getKeypoints frame by frame (example_bimp.cpp:48-54, cuda_bimp.cu:109-140); a batch

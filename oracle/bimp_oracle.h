@@ -111,6 +111,23 @@ int bo_direct_taps(const bo_scale* s, float nsig, int fp16, int* ext, double* fs
  * over the periodic padded image, 8 maps of fftH x fftW as bo_filter_scale. */
 int bo_direct_filter_scale(bo_ctx* c, int i, float nsig, int fp16, float* maps);
 
+/* Colour histogram of keypoints, addColorFeatures_kernel (cuda_bimp.cuh:8-71): keypoint i of
+ * the 7-row SURF_CUDA matrix kp (leading dimension ld) gets 24 floats at desc + i*desc_ld:
+ * 8 bins of B, then G, then R over the pixels (ix, iy) with ix in [(int)x - p, (int)x + p),
+ * iy likewise, p = (int)(log2(size) / 2), inside the image.  A channel byte is read as a
+ * signed char v and counts in bin k when 31.875*k < v < 31.875*(k+1) (so bytes 0 and >= 128
+ * count nowhere); bins are then scaled by 0.5 / (3 * pixels inside), in double.  The
+ * reference also reads column `width` and row `height` (its bounds are inclusive), outside
+ * the image; they are skipped here. */
+int bo_color_features(const float* kp, int ld, int n, const uint8_t* bgr, size_t pitch,
+                      int width, int height, float* desc, int desc_ld);
+/* Brute-force 2-nearest-neighbour matching by L2 distance (cv::cuda BFMatcher knnMatch with
+ * k = 2, bimp.cpp:188-205): for each query row, the train rows with the smallest and second
+ * smallest distance sqrt(sum (q - t)^2), ties to the lower train index; sums in FP64.
+ * idx2 / dist2 receive 2 entries per query (index -1 and +inf when there are fewer rows). */
+int bo_knn2(const float* query, int nq, int q_ld, const float* train, int nt, int t_ld, int dim,
+            int* idx2, float* dist2);
+
 #ifdef __cplusplus
 }
 #endif

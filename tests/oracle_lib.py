@@ -70,6 +70,9 @@ def lib():
                                      C.POINTER(C.c_double), P, P]
         L.bo_direct_filter_scale.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
         L.bimp_synth_frame.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, P, C.c_size_t]
+        L.bo_color_features.argtypes = [P, C.c_int, C.c_int, P, C.c_size_t, C.c_int, C.c_int, P,
+                                        C.c_int]
+        L.bo_knn2.argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P, P]
         _lib = L
     return _lib
 
@@ -82,6 +85,27 @@ def synth_frame(w, h, nshapes, seed):
     out = np.zeros((h, w), np.uint8)
     assert lib().bimp_synth_frame(w, h, nshapes, seed, ptr(out), w) == 0
     return out
+
+
+def color_features(kp_matrix, bgr):
+    """Oracle colour histogram: (n, 24) floats for a 7 x n keypoint matrix and a BGR frame."""
+    mat = np.ascontiguousarray(kp_matrix, np.float32)
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    h, w, _ = bgr.shape
+    n = mat.shape[1]
+    out = np.zeros((n, 24), np.float32)
+    assert lib().bo_color_features(ptr(mat), n, n, ptr(bgr), 3 * w, w, h, ptr(out), 24) == 0
+    return out
+
+
+def knn2(query, train):
+    q = np.ascontiguousarray(query, np.float32)
+    t = np.ascontiguousarray(train, np.float32)
+    nq, dim = q.shape
+    idx = np.zeros((nq, 2), np.int32)
+    dist = np.zeros((nq, 2), np.float32)
+    assert lib().bo_knn2(ptr(q), nq, dim, ptr(t), t.shape[0], dim, dim, ptr(idx), ptr(dist)) == 0
+    return idx, dist
 
 
 def snap(n):
