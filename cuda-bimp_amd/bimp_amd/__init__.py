@@ -82,6 +82,8 @@ EXPORTED = [
     "bimp_diff_forward", "bimp_diff_backward", "bimp_diff_last_error", "bimp_bgr_to_gray",
     "bimp_keypoints_cpu_mode", "bimp_time_detect_frames", "bimp_get_level",
     "bimp_color_features", "bimp_match_knn2", "bimp_ratio_matches",
+    "bimp_min_size_shape", "bimp_resize_linear_u8", "bimp_get_gpu_image", "bimp_map_view",
+    "bimp_memcpy_h2d_2d",
 ]
 
 
@@ -142,6 +144,11 @@ def _load():
         "bimp_time_detect_frames": (I, [P, P, SZ, P, I, I, P, P]),
         "bimp_get_level": (I, [P, I, P, SZ, P]),
         "bimp_color_features": (I, [P, I, I, P, SZ, I, I, P, I, P]),
+        "bimp_min_size_shape": (I, [I, I, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_float)]),
+        "bimp_resize_linear_u8": (I, [P, SZ, I, I, I, P, SZ, I, I, C.c_double, C.c_double, P]),
+        "bimp_get_gpu_image": (I, [P, SZ, I, I, I, I, P, SZ, P]),
+        "bimp_memcpy_h2d_2d": (I, [P, SZ, P, SZ, SZ, SZ]),
+        "bimp_map_view": (I, [P, I, I, I, C.POINTER(P), C.POINTER(SZ), C.POINTER(I), C.POINTER(I)]),
         "bimp_match_knn2": (I, [P, I, I, P, I, I, I, P, P, P]),
         "bimp_ratio_matches": (I, [P, P, I, C.c_double, P, I, C.POINTER(I)]),
         "bimp_synth_frame": (I, [I, I, I, C.c_uint64, P, SZ]),
@@ -394,6 +401,44 @@ def bgr_to_gray(bgr):
     out = np.zeros((h, w), np.uint8)
     d.download(out)
     d.free()
+    return out
+
+
+def min_size_shape(width, height):
+    """(width, height, factor) of getGPUImage's prepared frame (cuda_bimp.cu:156-168)."""
+    w, h, f = C.c_int(0), C.c_int(0), C.c_float(0)
+    _check(lib.bimp_min_size_shape(int(width), int(height), C.byref(w), C.byref(h), C.byref(f)))
+    return w.value, h.value, f.value
+
+
+def get_gpu_image(img, grayscale=True):
+    """getGPUImage on the device: the minimum-size upscale of a uint8 frame (H, W) or BGR
+    (H, W, 3), then BGR -> gray when grayscale; returns the prepared host copy."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    cn = 1 if img.ndim == 2 else img.shape[2]
+    ow, oh, _ = min_size_shape(w, h)
+    ocn = 1 if (grayscale and cn == 3) else cn
+    out = np.zeros((oh, ow) if ocn == 1 else (oh, ow, ocn), np.uint8)
+    ds, dd = DeviceBuffer(img.nbytes), DeviceBuffer(out.nbytes)
+    ds.upload(img)
+    _check(lib.bimp_get_gpu_image(ds.ptr, w * cn, w, h, cn, 1 if grayscale else 0, dd.ptr,
+                                  ow * ocn, None))
+    dd.download(out)
+    return out
+
+
+def resize_linear(img, dw, dh, fx=0.0, fy=0.0):
+    """Bilinear resize (cv::resize INTER_LINEAR semantics) of a uint8 frame on the device."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    cn = 1 if img.ndim == 2 else img.shape[2]
+    out = np.zeros((dh, dw) if cn == 1 else (dh, dw, cn), np.uint8)
+    ds, dd = DeviceBuffer(img.nbytes), DeviceBuffer(out.nbytes)
+    ds.upload(img)
+    _check(lib.bimp_resize_linear_u8(ds.ptr, w * cn, w, h, cn, dd.ptr, dw * cn, dw, dh,
+                                     float(fx), float(fy), None))
+    dd.download(out)
     return out
 
 

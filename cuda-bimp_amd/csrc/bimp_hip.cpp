@@ -853,6 +853,27 @@ int bimp_get_map(bimp_ctx* c, int i, int which, int o, float* dst, size_t pitch,
     return BIMP_OK;
 }
 
+int bimp_map_view(const bimp_ctx* c, int i, int which, int o, const float** d_map, size_t* pitch,
+                  int* rows, int* cols) {
+    if (!c || !d_map || !pitch || !rows || !cols || i < 0 || i >= (int)c->sc.size())
+        return BIMP_E_ARG;
+    const auto& s = c->sc[i];
+    if (which == BIMP_MAP_DOUBLE || which == BIMP_MAP_SINGLE) {
+        *d_map = which == BIMP_MAP_DOUBLE ? s.D : s.S;
+        *rows = s.g.H;
+        *cols = s.g.W;
+        *pitch = (size_t)s.g.W;
+    } else if (which == BIMP_MAP_COMPLEX && o >= 0 && o < kNumOri) {
+        *d_map = s.C + (size_t)o * s.Hc * s.ldc;
+        *rows = s.Hc;
+        *cols = s.Wc;
+        *pitch = (size_t)s.ldc;
+    } else {
+        return BIMP_E_ARG;
+    }
+    return BIMP_OK;
+}
+
 int bimp_set_map(bimp_ctx* c, int i, int which, int o, const float* src, size_t pitch,
                  int src_on_device, bimp_stream_t stream) {
     if (!c || !src || i < 0 || i >= (int)c->sc.size()) return BIMP_E_ARG;
@@ -971,6 +992,29 @@ int bimp_device_malloc(void** p, size_t bytes) {
 int bimp_device_free(void* p) { return hipFree(p) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE; }
 int bimp_memcpy_h2d(void* d, const void* s, size_t b) {
     return hipMemcpy(d, s, b, hipMemcpyHostToDevice) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;
+}
+int bimp_memcpy_h2d_2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                       size_t width_bytes, size_t rows) {
+    if (!dst || !src || dpitch < width_bytes || spitch < width_bytes) return BIMP_E_ARG;
+    if (!rows || !width_bytes) return BIMP_OK;
+    // one contiguous upload: HIP's pitched copy from pageable memory is many times slower
+    std::vector<uint8_t> packed;
+    const uint8_t* s8 = static_cast<const uint8_t*>(src);
+    if (spitch != width_bytes) {
+        packed.resize(width_bytes * rows);
+        for (size_t y = 0; y < rows; ++y)
+            std::memcpy(packed.data() + y * width_bytes, s8 + y * spitch, width_bytes);
+        s8 = packed.data();
+    }
+    if (dpitch == width_bytes)
+        return hipMemcpy(dst, s8, width_bytes * rows, hipMemcpyHostToDevice) == hipSuccess
+                   ? BIMP_OK
+                   : BIMP_E_DEVICE;
+    for (size_t y = 0; y < rows; ++y)
+        if (hipMemcpy(static_cast<uint8_t*>(dst) + y * dpitch, s8 + y * width_bytes, width_bytes,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return BIMP_E_DEVICE;
+    return BIMP_OK;
 }
 int bimp_memcpy_d2h(void* d, const void* s, size_t b) {
     return hipMemcpy(d, s, b, hipMemcpyDeviceToHost) == hipSuccess ? BIMP_OK : BIMP_E_DEVICE;

@@ -2,9 +2,9 @@
 //
 // Mirrors daniel-koudouna/cuda-bimp's host interface (bimp.hpp:19-51 Context,
 // cuda_bimp.hpp:92-146 CudaStuff / getKeypoints / makeLambdasLog / snapTransformSize,
-// utils.hpp:32 downloadKeypoints) over the C ABI in bimp_hip.h.  Frames are 8-bit grayscale;
-// keypoints come back in cv::KeyPoint field order, and the 7-row SURF_CUDA matrix layout is
-// available for callers that consumed the reference's GpuMat.  When OpenCV's core header is
+// utils.hpp:32 downloadKeypoints) over the C ABI in bimp_hip.h.  Frames are 8-bit gray or BGR;
+// keypoints come back in cv::KeyPoint field order, and the 7-row SURF_CUDA matrix is kept on
+// the device for callers that consumed the reference's GpuMat.  When OpenCV's core header is
 // on the include path, cv::Mat / cv::KeyPoint overloads are compiled as well.
 //
 // Errors: the reference exit()s inside its CUDA error macros; this facade throws
@@ -125,47 +125,143 @@ inline int getKeypoints(const uint8_t* img, size_t pitch, std::vector<float>& ma
 
 }  // namespace cuda
 
-// Context (bimp.hpp:19-51, bimp.cpp:7-129), grayscale frames.  The reference constructor
-// uses makeLambdasLog(4, 64, 2) (bimp.cpp:8); a bank can be given explicitly.
+// Device memory owned by the facade (freed on destruction).
+class DeviceBytes {
+public:
+    DeviceBytes() = default;
+    explicit DeviceBytes(size_t n) { reset(n); }
+    ~DeviceBytes() {
+        if (p_) bimp_device_free(p_);
+    }
+    DeviceBytes(const DeviceBytes&) = delete;
+    DeviceBytes& operator=(const DeviceBytes&) = delete;
+    void reset(size_t n) {
+        if (n <= n_) return;
+        if (p_) bimp_device_free(p_);
+        p_ = nullptr;
+        n_ = 0;
+        check(bimp_device_malloc(&p_, n));
+        n_ = n;
+    }
+    template <class T = uint8_t>
+    T* get() const { return static_cast<T*>(p_); }
+    size_t size() const { return n_; }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Context (bimp.hpp:19-51, bimp.cpp:7-129).  The reference constructor uses
+// makeLambdasLog(4, 64, 2) (bimp.cpp:8); a bank can be given explicitly.  Frames are 8-bit
+// gray (channels 1) or BGR (channels 3).  As getGPUImage does (cuda_bimp.cu:156-177), a frame
+// whose rows (with cols >= 350) or cols are below 350 is upscaled bilinearly on the device and
+// a colour frame is converted to gray there; the CudaStuff keeps the size of the frame given
+// to the constructor (bimp.cpp:8-10), so detection reads the top-left width() x height() of the
+// prepared frame, as the reference's does.
 class Context {
 public:
+    // Device view of one scale's response maps (CudaKernels, cuda_bimp.hpp:38-60): the
+    // context's own device memory, as Context::getResponses hands out (bimp.cpp:94-104).
+    struct ResponseView {
+        bimp_scale_info info;
+        const float* d_double = nullptr;  // data_h x data_w, pitch floats per row
+        const float* d_single = nullptr;
+        size_t pitch = 0;
+    };
+    // Host copies of one scale's response maps (downloadResultsToCPU, bimp.cpp:75-80).
     struct Response {
         bimp_scale_info info;
         std::vector<float> doubles, singles;  // data_h x data_w maps (d_ResultDouble/Single)
     };
+    // The 7-row SURF_CUDA keypoint matrix on the device (getKeypoints' GpuMat, bimp.hpp:36):
+    // row r of keypoint i at d_matrix[r * ld + i], count columns.
+    struct DeviceKeypoints {
+        const float* d_matrix = nullptr;
+        int ld = 0;
+        int count = 0;
+    };
 
-    Context(const uint8_t* gray, int width, int height, size_t pitch,
+    Context(const uint8_t* img, int width, int height, size_t pitch,
             std::vector<float> lambdas = cuda::makeLambdasLog(4, 64, 2),
-            const bimp_options* opt = nullptr)
+            const bimp_options* opt = nullptr, int channels = 1)
         : cuda_data(width, height, lambdas, opt), lambdas_(std::move(lambdas)),
           width_(width), height_(height) {
-        loadImage(gray, pitch);
+        loadImage(img, width, height, pitch, channels);
     }
 
     int width() const { return width_; }
     int height() const { return height_; }
+    // Size of the prepared frame detection reads from (>= width() x height()).
+    int frameWidth() const { return frame_w_; }
+    int frameHeight() const { return frame_h_; }
 
-    // The frame is copied: the reference keeps its own GpuMat (bimp.cpp:119-123).
-    void loadImage(const uint8_t* gray, size_t pitch) {
-        image_.resize((size_t)width_ * height_);
-        for (int y = 0; y < height_; ++y)
-            for (int x = 0; x < width_; ++x)
-                image_[(size_t)y * width_ + x] = gray[(size_t)y * pitch + x];
+    // loadImage(Mat) (bimp.cpp:106-109): a host frame, prepared as by getGPUImage.
+    void loadImage(const uint8_t* img, size_t pitch) { loadImage(img, width_, height_, pitch, 1); }
+    void loadImage(const uint8_t* img, int w, int h, size_t pitch, int channels) {
+        if (channels != 1 && channels != 3) check(BIMP_E_ARG);
+        int pw = 0, ph = 0;
+        float sf = 1.f;
+        check(bimp_min_size_shape(w, h, &pw, &ph, &sf));
+        if (pw < width_ || ph < height_) check(BIMP_E_ARG);
+        const size_t row = (size_t)w * channels;
+        raw_.reset(row * h);
+        check(bimp_memcpy_h2d_2d(raw_.get(), row, img, pitch, row, h));
+        image_.reset((size_t)pw * ph);
+        check(bimp_get_gpu_image(raw_.get(), row, w, h, channels, 1, image_.get(), (size_t)pw,
+                                 nullptr));
+        if (channels == 3) {
+            color_.reset((size_t)pw * ph * 3);
+            check(bimp_get_gpu_image(raw_.get(), row, w, h, 3, 0, color_.get(), (size_t)pw * 3,
+                                     nullptr));
+        }
+        check(bimp_stream_sync(nullptr));
+        d_image_ = image_.get();
+        image_pitch_ = (size_t)pw;
+        frame_w_ = pw;
+        frame_h_ = ph;
+        has_color_ = channels == 3;
+        has_changed_keypoints = true;
+    }
+    // loadImage(GpuMat) (bimp.cpp:111-117): a device gray frame used as it is (no resize); it
+    // must stay valid while the context uses it and be at least width() x height().
+    void loadImageDevice(const uint8_t* d_gray, int w, int h, size_t pitch) {
+        if (!d_gray || w < width_ || h < height_ || pitch < (size_t)w) check(BIMP_E_ARG);
+        d_image_ = d_gray;
+        image_pitch_ = pitch;
+        frame_w_ = w;
+        frame_h_ = h;
+        has_color_ = false;
         has_changed_keypoints = true;
     }
 
-    // 7-row SURF_CUDA keypoint matrix (leading dimension = number of keypoints)
-    const std::vector<float>& getKeypoints() {
+    // getKeypoints (bimp.cpp:61-73): the device keypoint matrix, at most KP_MAX_POINTS columns
+    // (overflow warns on stderr and clamps, cuda_bimp.cu:132-138).
+    const DeviceKeypoints& getKeypoints() {
         refresh();
-        return matrix_;
+        return device_kp_;
     }
-
+    // getKeypointsAndDownload (bimp.cpp:89-92)
     std::vector<KeyPoint> getKeypointsAndDownload() {
         refresh();
         return keypoints_;
     }
 
-    std::vector<Response> getResponses() {
+    std::vector<ResponseView> getResponses() const {
+        std::vector<ResponseView> out;
+        for (int i = 0; i < (int)cuda_data.ks.size(); ++i) {
+            ResponseView r;
+            r.info = cuda_data.ks[(size_t)i];
+            int rows = 0, cols = 0;
+            check(bimp_map_view(cuda_data.ctx, i, BIMP_MAP_DOUBLE, 0, &r.d_double, &r.pitch,
+                                &rows, &cols), cuda_data.ctx);
+            check(bimp_map_view(cuda_data.ctx, i, BIMP_MAP_SINGLE, 0, &r.d_single, &r.pitch,
+                                &rows, &cols), cuda_data.ctx);
+            out.push_back(r);
+        }
+        return out;
+    }
+    std::vector<Response> downloadResultsToCPU() {
         std::vector<Response> out;
         for (int i = 0; i < (int)cuda_data.ks.size(); ++i) {
             Response r;
@@ -181,6 +277,23 @@ public:
         return out;
     }
 
+    // The colour histogram part of getDescriptors (bimp.cpp:32-59): BIMP_COLOR_DIMENSIONS
+    // floats per keypoint of a colour frame, computed on the device and downloaded.
+    std::vector<float> getColorFeaturesAndDownload() {
+        refresh();
+        if (!has_color_) check(BIMP_E_ARG);
+        const int n = device_kp_.count;
+        std::vector<float> out((size_t)n * BIMP_COLOR_DIMENSIONS);
+        if (n == 0) return out;
+        desc_.reset(out.size() * sizeof(float));
+        check(bimp_color_features(device_kp_.d_matrix, device_kp_.ld, n, color_.get(),
+                                  (size_t)frame_w_ * 3, frame_w_, frame_h_, desc_.get<float>(),
+                                  BIMP_COLOR_DIMENSIONS, nullptr));
+        check(bimp_stream_sync(nullptr));
+        check(bimp_memcpy_d2h(out.data(), desc_.get(), out.size() * sizeof(float)));
+        return out;
+    }
+
     // Context::setCPUMode (bimp.hpp:43-44): keypoints from the reference's CPU extraction rules
     // (bimp_keypoints_cpu_mode) instead of the device NMS.
     void setCPUMode(bool cpu_mode) {
@@ -190,9 +303,12 @@ public:
     bool getCPUMode() const { return cpu_mode_; }
 
 #ifdef BIMP_HAVE_OPENCV
-    explicit Context(const cv::Mat& gray, std::vector<float> lambdas = cuda::makeLambdasLog(4, 64, 2))
-        : Context(gray.ptr<uint8_t>(), gray.cols, gray.rows, gray.step, std::move(lambdas)) {}
-    void loadImage(const cv::Mat& gray) { loadImage(gray.ptr<uint8_t>(), gray.step); }
+    explicit Context(const cv::Mat& image, std::vector<float> lambdas = cuda::makeLambdasLog(4, 64, 2))
+        : Context(image.ptr<uint8_t>(), image.cols, image.rows, image.step, std::move(lambdas),
+                  nullptr, image.channels()) {}
+    void loadImage(const cv::Mat& image) {
+        loadImage(image.ptr<uint8_t>(), image.cols, image.rows, image.step, image.channels());
+    }
     std::vector<cv::KeyPoint> getKeypointsAndDownloadCV() {
         std::vector<cv::KeyPoint> out;
         for (const KeyPoint& k : getKeypointsAndDownload())
@@ -207,28 +323,52 @@ public:
 private:
     void refresh() {
         if (!has_changed_keypoints) return;
+        const int cap = BIMP_KP_MAX_POINTS;
         if (cpu_mode_) {
-            check(bimp_filter_u8(cuda_data.ctx, image_.data(), (size_t)width_, 0, nullptr),
-                  cuda_data.ctx);
+            check(bimp_filter_u8(cuda_data.ctx, d_image_, image_pitch_, 1, nullptr), cuda_data.ctx);
             check(bimp_cells(cuda_data.ctx, nullptr), cuda_data.ctx);
-            keypoints_.resize((size_t)BIMP_KP_MAX_POINTS);
+            keypoints_.resize((size_t)cap);
             int count = 0;
-            const int st = bimp_keypoints_cpu_mode(cuda_data.ctx, keypoints_.data(),
-                                                   BIMP_KP_MAX_POINTS, &count, nullptr);
-            check(st, cuda_data.ctx);
-            keypoints_.resize((size_t)(count < BIMP_KP_MAX_POINTS ? count : BIMP_KP_MAX_POINTS));
+            check(bimp_keypoints_cpu_mode(cuda_data.ctx, keypoints_.data(), cap, &count, nullptr),
+                  cuda_data.ctx);
+            if (count > cap) warn(count);
+            keypoints_.resize((size_t)(count < cap ? count : cap));
+            matrix_d_.reset((size_t)BIMP_KP_VECTOR_LENGTH * cap * sizeof(float));
+            std::vector<float> mat((size_t)BIMP_KP_VECTOR_LENGTH * cap, 0.f);
+            check(bimp_keypoints_to_matrix(keypoints_.data(), (int)keypoints_.size(), mat.data(), cap));
+            check(bimp_memcpy_h2d(matrix_d_.get(), mat.data(), mat.size() * sizeof(float)));
         } else {
-            keypoints_ = cuda::getKeypoints(image_.data(), (size_t)width_, cuda_data);
+            matrix_d_.reset((size_t)BIMP_KP_VECTOR_LENGTH * cap * sizeof(float));
+            int count = 0;
+            const int st = bimp_detect_u8_matrix(cuda_data.ctx, d_image_, image_pitch_, 1,
+                                                 matrix_d_.get<float>(), cap, cap, 1, &count,
+                                                 nullptr);
+            check(st, cuda_data.ctx);
+            if (st == BIMP_W_OVERFLOW) warn(count);
+            const int n = count < cap ? count : cap;
+            std::vector<float> mat((size_t)BIMP_KP_VECTOR_LENGTH * cap);
+            check(bimp_memcpy_d2h(mat.data(), matrix_d_.get(), mat.size() * sizeof(float)));
+            keypoints_ = utils::downloadKeypoints(mat.data(), cap, n);
         }
-        matrix_ = utils::uploadKeypoints(keypoints_);
+        device_kp_.d_matrix = matrix_d_.get<float>();
+        device_kp_.ld = cap;
+        device_kp_.count = (int)keypoints_.size();
         has_changed_keypoints = false;
+    }
+    static void warn(int count) {
+        std::fprintf(stderr, "Warning: Keypoints (%d) exceed current keypoint maximum (%d).\n",
+                     count, BIMP_KP_MAX_POINTS);
     }
     bool cpu_mode_ = false;
     std::vector<float> lambdas_;
     int width_ = 0, height_ = 0;
-    std::vector<uint8_t> image_;
+    DeviceBytes raw_, image_, color_, matrix_d_, desc_;
+    const uint8_t* d_image_ = nullptr;
+    size_t image_pitch_ = 0;
+    int frame_w_ = 0, frame_h_ = 0;
+    bool has_color_ = false;
     std::vector<KeyPoint> keypoints_;
-    std::vector<float> matrix_;
+    DeviceKeypoints device_kp_;
 };
 
 }  // namespace bimp

@@ -173,6 +173,11 @@ int bimp_keypoints(bimp_ctx* ctx, bimp_keypoint* out, int cap, int out_on_device
  * COMPLEX maps are map_h x map_w.  pitch is in floats.  Synchronous on `stream`. */
 int bimp_get_map(bimp_ctx* ctx, int scale, int which, int orientation, float* dst,
                  size_t pitch, int dst_on_device, bimp_stream_t stream);
+/* Device view of the same map (Context::getResponses hands out the context's own device
+ * pointers, bimp.cpp:94-104): pointer, pitch in floats, rows and columns.  The view stays valid
+ * until the context is destroyed; its contents change with each frame. */
+int bimp_map_view(const bimp_ctx* ctx, int scale, int which, int orientation, const float** d_map,
+                  size_t* pitch, int* rows, int* cols);
 int bimp_set_map(bimp_ctx* ctx, int scale, int which, int orientation, const float* src,
                  size_t pitch, int src_on_device, bimp_stream_t stream);
 
@@ -207,6 +212,9 @@ int bimp_device_count(int* n);
 int bimp_device_malloc(void** p, size_t bytes);
 int bimp_device_free(void* p);
 int bimp_memcpy_h2d(void* dst, const void* src, size_t bytes);
+/* rows x width_bytes from pitched host memory to pitched device memory (synchronous). */
+int bimp_memcpy_h2d_2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                       size_t width_bytes, size_t rows);
 int bimp_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int bimp_stream_create(bimp_stream_t* s);
 int bimp_stream_destroy(bimp_stream_t s);
@@ -248,6 +256,29 @@ int bimp_keypoints_cpu_mode(bimp_ctx* ctx, bimp_keypoint* out, int cap, int* cou
  * The MIN_SIZE upscale of getGPUImage is not provided. */
 int bimp_bgr_to_gray(const uint8_t* bgr, size_t bgr_pitch, int width, int height, int bgr_on_device,
                      uint8_t* d_gray, size_t gray_pitch, bimp_stream_t stream);
+
+/* ---- Frame preparation (getGPUImage, cuda_bimp.cu:156-177) ------------------------------
+ * The reference upscales a frame with rows < 350 <= cols by 350/rows, else one with
+ * cols < 350 by 350/cols (a float factor), with cv::resize's bilinear filter, then converts
+ * BGR to gray on the device.  A Context keeps its CudaStuff sized from the frame before the
+ * upscale (bimp.cpp:8-10), so detection reads the top-left width x height of the upscaled
+ * frame: pass the prepared frame and its pitch to bimp_detect_u8 with the original size. */
+/* Size of the prepared frame (the input size when no upscale applies) and the factor (1 when
+ * none). */
+int bimp_min_size_shape(int width, int height, int* out_w, int* out_h, float* scale);
+/* Bilinear resize of a device 8-bit frame of 1 or 3 interleaved channels (cv::resize
+ * INTER_LINEAR: source coordinate (x + 0.5) / fx - 0.5 clamped to the frame, 11-bit fixed-point
+ * weights, one rounding of the two-axis sum).  fx, fy > 0 give the factors (as cv::resize's
+ * fx, fy); otherwise the factors are dw/sw and dh/sh.  No synchronisation. */
+int bimp_resize_linear_u8(const uint8_t* d_src, size_t src_pitch, int sw, int sh, int channels,
+                          uint8_t* d_dst, size_t dst_pitch, int dw, int dh, double fx, double fy,
+                          bimp_stream_t stream);
+/* getGPUImage on a device frame: the minimum-size upscale, then BGR -> gray when channels is
+ * 3 and grayscale is set (otherwise the channels are kept).  d_dst holds the prepared frame
+ * (bimp_min_size_shape gives its size).  No synchronisation. */
+int bimp_get_gpu_image(const uint8_t* d_src, size_t src_pitch, int width, int height,
+                       int channels, int grayscale, uint8_t* d_dst, size_t dst_pitch,
+                       bimp_stream_t stream);
 
 /* ---- Descriptors and matching (the keypoints' consumers) --------------------------------
  * Context::getDescriptors (bimp.cpp:32-59) appends BIMP_COLOR_DIMENSIONS colour-histogram

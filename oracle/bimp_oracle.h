@@ -121,6 +121,15 @@ int bo_direct_filter_scale(bo_ctx* c, int i, float nsig, int fp16, float* maps);
  * the image; they are skipped here. */
 int bo_color_features(const float* kp, int ld, int n, const uint8_t* bgr, size_t pitch,
                       int width, int height, float* desc, int desc_ld);
+/* getGPUImage's minimum-size upscale (cuda_bimp.cu:156-168): cv::resize INTER_LINEAR of an
+ * 8-bit frame (cn interleaved channels) by factors fx, fy, restated from its published
+ * definition (OpenCV 3.3.1; parity with OpenCV itself unpinned): per axis the source
+ * coordinate f = (float)((d + 0.5) / factor - 0.5), s = floor(f), fraction f - s (0 and s
+ * clamped at either end), weights round((1 - frac) * 2048) and round(frac * 2048) (each to
+ * nearest even), value = (wy0 * (wx0 * p00 + wx1 * p01) + wy1 * (wx0 * p10 + wx1 * p11)
+ * + 2^21) >> 22. */
+int bo_resize_linear(const uint8_t* src, size_t spitch, int sw, int sh, int cn, uint8_t* dst,
+                     size_t dpitch, int dw, int dh, double fx, double fy);
 /* Brute-force 2-nearest-neighbour matching by L2 distance (cv::cuda BFMatcher knnMatch with
  * k = 2, bimp.cpp:188-205): for each query row, the train rows with the smallest and second
  * smallest distance sqrt(sum (q - t)^2), ties to the lower train index; sums in FP64.

@@ -72,6 +72,8 @@ def lib():
         L.bimp_synth_frame.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, P, C.c_size_t]
         L.bo_color_features.argtypes = [P, C.c_int, C.c_int, P, C.c_size_t, C.c_int, C.c_int, P,
                                         C.c_int]
+        L.bo_resize_linear.argtypes = [P, C.c_size_t, C.c_int, C.c_int, C.c_int, P, C.c_size_t,
+                                       C.c_int, C.c_int, C.c_double, C.c_double]
         L.bo_knn2.argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P, P]
         _lib = L
     return _lib
@@ -95,6 +97,15 @@ def color_features(kp_matrix, bgr):
     n = mat.shape[1]
     out = np.zeros((n, 24), np.float32)
     assert lib().bo_color_features(ptr(mat), n, n, ptr(bgr), 3 * w, w, h, ptr(out), 24) == 0
+    return out
+
+
+def resize_linear(img, dw, dh, fx, fy):
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    cn = 1 if img.ndim == 2 else img.shape[2]
+    out = np.zeros((dh, dw) if cn == 1 else (dh, dw, cn), np.uint8)
+    assert lib().bo_resize_linear(ptr(img), w * cn, w, h, cn, ptr(out), dw * cn, dw, dh, fx, fy) == 0
     return out
 
 

@@ -117,3 +117,66 @@ def test_ratio_matches_known_answers():
     assert list(m["train_idx"]) == [4, 9, 5]
     assert np.array_equal(m["distance"], np.float32([1.0, 0.59, 0.1]))
     assert np.all(m["img_idx"] == 0)
+
+
+# ------------------------------------------------------------------ frame preparation
+def _shape_rule(w, h):
+    """getGPUImage (cuda_bimp.cu:159-168) and cv::resize's output size, from the text."""
+    if h < 350 and w >= 350:
+        sf = np.float32(350 / h)
+    elif w < 350:
+        sf = np.float32(350 / w)
+    else:
+        return w, h
+    return int(np.rint(w * np.float64(sf))), int(np.rint(h * np.float64(sf)))
+
+
+def test_min_size_shape_rule():
+    for w in list(range(1, 800, 7)) + [349, 350, 351]:
+        for h in list(range(1, 800, 11)) + [349, 350, 351]:
+            ow, oh, f = B.min_size_shape(w, h)
+            assert (ow, oh) == _shape_rule(w, h), (w, h)
+            assert (f == 1.0) == ((ow, oh) == (w, h))
+
+
+def test_resize_known_answers():
+    # constant frames stay constant
+    c = np.full((30, 40), 137, np.uint8)
+    assert np.all(O.resize_linear(c, 77, 59, 77 / 40, 59 / 30) == 137)
+    # factor 2: output x maps to source (x + 0.5) / 2 - 0.5, so pairs of outputs sit a quarter
+    # pixel either side of a source sample (weights 1536/512), clamped at the ends
+    row = np.array([[0, 128, 255, 64]], np.uint8)
+    out = O.resize_linear(np.repeat(row, 3, 0), 8, 6, 2.0, 2.0)[0]
+    want = [0, 32, 96, 160, 223, 207, 112, 64]
+    assert list(out) == want
+    # channels are independent
+    rgb = np.stack([np.repeat(row, 3, 0), np.repeat(row, 3, 0) // 2, np.full((3, 4), 9, np.uint8)], -1)
+    o3 = O.resize_linear(rgb, 8, 6, 2.0, 2.0)
+    assert list(o3[0, :, 0]) == want and np.all(o3[..., 2] == 9)
+
+
+def test_resize_close_to_float_bilinear():
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (120, 160), dtype=np.uint8)
+    sf = np.float32(350 / 160)
+    dw, dh = _shape_rule(160, 120)
+    got = O.resize_linear(img, dw, dh, float(sf), float(sf)).astype(np.int32)
+    inv = 1.0 / np.float64(sf)
+
+    def coords(n_out, n_in):
+        f = (np.arange(n_out) + 0.5) * inv - 0.5
+        s = np.floor(f).astype(int)
+        fr = f - s
+        fr[s < 0] = 0
+        s[s < 0] = 0
+        fr[s >= n_in - 1] = 0
+        s[s >= n_in - 1] = n_in - 1
+        return s, np.minimum(s + 1, n_in - 1), fr
+
+    x0, x1, fx = coords(dw, 160)
+    y0, y1, fy = coords(dh, 120)
+    im = img.astype(np.float64)
+    top = im[y0][:, x0] * (1 - fx) + im[y0][:, x1] * fx
+    bot = im[y1][:, x0] * (1 - fx) + im[y1][:, x1] * fx
+    ref = top * (1 - fy)[:, None] + bot * fy[:, None]
+    assert np.abs(got - ref).max() <= 1.0
