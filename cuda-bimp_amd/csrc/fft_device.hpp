@@ -84,6 +84,36 @@ __device__ __forceinline__ void dft5(float2* v) {
     v[3] = csub(A2, mul_mi<INV>(B2));
 }
 
+// X_q = sum_k x_k w^(qk), w = exp(-+2*pi*i/7): with p_k = x_k + x_{7-k} and m_k = x_k - x_{7-k}
+// (k = 1..3), X_q = x_0 + sum_k cos(2*pi*q*k/7) p_k -+ i sum_k sin(2*pi*q*k/7) m_k and
+// X_{7-q} its partner with the opposite sign of the sine sum.
+template <bool INV>
+__device__ __forceinline__ void dft7(float2* v) {
+    const float c1 = 0.62348980185873353f, c2 = -0.22252093395631440f, c3 = -0.90096886790241913f;
+    const float s1 = 0.78183148246802981f, s2 = 0.97492791218182361f, s3 = 0.43388373911755812f;
+    const float2 a = v[0];
+    const float2 p1 = cadd(v[1], v[6]), m1 = csub(v[1], v[6]);
+    const float2 p2 = cadd(v[2], v[5]), m2 = csub(v[2], v[5]);
+    const float2 p3 = cadd(v[3], v[4]), m3 = csub(v[3], v[4]);
+    // cos(2 pi q k / 7) and sin(2 pi q k / 7) for q, k = 1..3 reduce to c1..c3, +-s1..s3
+    const float2 A1 = make_float2(a.x + c1 * p1.x + c2 * p2.x + c3 * p3.x,
+                                  a.y + c1 * p1.y + c2 * p2.y + c3 * p3.y);
+    const float2 A2 = make_float2(a.x + c2 * p1.x + c3 * p2.x + c1 * p3.x,
+                                  a.y + c2 * p1.y + c3 * p2.y + c1 * p3.y);
+    const float2 A3 = make_float2(a.x + c3 * p1.x + c1 * p2.x + c2 * p3.x,
+                                  a.y + c3 * p1.y + c1 * p2.y + c2 * p3.y);
+    const float2 B1 = make_float2(s1 * m1.x + s2 * m2.x + s3 * m3.x, s1 * m1.y + s2 * m2.y + s3 * m3.y);
+    const float2 B2 = make_float2(s2 * m1.x - s3 * m2.x - s1 * m3.x, s2 * m1.y - s3 * m2.y - s1 * m3.y);
+    const float2 B3 = make_float2(s3 * m1.x - s1 * m2.x + s2 * m3.x, s3 * m1.y - s1 * m2.y + s2 * m3.y);
+    v[0] = cadd(a, cadd(p1, cadd(p2, p3)));
+    v[1] = cadd(A1, mul_mi<INV>(B1));
+    v[6] = csub(A1, mul_mi<INV>(B1));
+    v[2] = cadd(A2, mul_mi<INV>(B2));
+    v[5] = csub(A2, mul_mi<INV>(B2));
+    v[3] = cadd(A3, mul_mi<INV>(B3));
+    v[4] = csub(A3, mul_mi<INV>(B3));
+}
+
 // w8^q * x for q = 1..3 (w8 = exp(-+i*pi/4))
 template <bool INV, int Q>
 __device__ __forceinline__ float2 w8(float2 x) {
@@ -144,6 +174,7 @@ __device__ __forceinline__ void dft(float2* v) {
     else if constexpr (R == 3) dft3<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v);
     else if constexpr (R == 5) dft5<INV>(v);
+    else if constexpr (R == 7) dft7<INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
     else if constexpr (R == 16) dft16<INV>(v);
     else static_assert(R == 2, "unsupported radix");
@@ -174,6 +205,8 @@ BIMP_FFT_CFG(1024, 64, 16, 16, 4)
 BIMP_FFT_CFG(1536, 64, 8, 8, 8, 3)
 BIMP_FFT_CFG(2048, 64, 16, 16, 8)
 BIMP_FFT_CFG(2560, 64, 8, 8, 8, 5)
+BIMP_FFT_CFG(3072, 64, 16, 16, 4, 3)
+BIMP_FFT_CFG(3584, 64, 8, 8, 8, 7)
 BIMP_FFT_CFG(4096, 64, 16, 16, 16)
 #undef BIMP_FFT_CFG
 

@@ -92,7 +92,8 @@ constexpr int kNmsBlockPixels = kNmsThreads * kNmsPerThread;
 bool fft_size_supported(int n);
 // Packed per-stage twiddle table of the length-n transform (FP64-evaluated, FP32-stored).
 // Transform lengths with a plan (fft_device.hpp): every kernel family is instantiated for each.
-#define BIMP_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(4096)
+#define BIMP_FOR_SIZES(X) \
+    X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(1536) X(2048) X(2560) X(3072) X(3584) X(4096)
 bool twiddle_table(int n, std::vector<float2>& out);
 // Register the dynamic-LDS limits of every kernel instance for size n (once per process).
 hipError_t prepare_size(int n);

@@ -198,3 +198,20 @@ def test_repeat_frames_identical(B):
     a, n1 = cs.detect(img)
     b, n2 = cs.detect(img)
     assert n1 == n2 and np.array_equal(bits(a), bits(b))
+
+
+@pytest.mark.parametrize("w,h,fw,fh", [(2600, 200, 3072, 256), (3100, 96, 3584, 128),
+                                       (48, 2600, 64, 3072), (40, 3100, 64, 3584)])
+def test_transform_sizes_3072_3584(B, w, h, fw, fh):
+    """Sides that snap to 3072 (16*16*4*3) and 3584 (8*8*8*7) in both directions."""
+    lam = O.BASE_BANK[:2]
+    cs, orc = _pair(B, w, h, lam)
+    assert (cs.ks[0].fft_w, cs.ks[0].fft_h) == (fw, fh)
+    img = _frame(w, h, 30, 19)
+    kps, n, Ds, Ss = orc.detect(img, want_maps=True)
+    got, gn = cs.detect(img)
+    for i in range(len(cs.ks)):
+        e = rel_err(cs.get_map(i, B.MAP_DOUBLE), Ds[i])
+        print(f"{w}x{h} scale {i} D rel err {e:.2e}", flush=True)
+        assert e < MAP_TOL
+    assert agreement(got, kps) >= 0.99
