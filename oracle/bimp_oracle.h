@@ -130,6 +130,14 @@ int bo_color_features(const float* kp, int ld, int n, const uint8_t* bgr, size_t
  * + 2^21) >> 22. */
 int bo_resize_linear(const uint8_t* src, size_t spitch, int sw, int sh, int cn, uint8_t* dst,
                      size_t dpitch, int dw, int dh, double fx, double fy);
+/* Integral image (exclusive: (H+1) x (W+1), first row and column 0) of an 8-bit frame. */
+int bo_integral_image(const uint8_t* img, size_t pitch, int width, int height, long* ii);
+/* SURF descriptors (64 floats, unit length) of keypoints of a 7-row matrix, following the
+ * published method (Bay et al. 2008) as the library states it (surf.hip): orientation from
+ * Haar responses in a disc of radius 6s and 72 windows of pi/3 (upright = 0), 4 x 4
+ * subregions of 5 x 5 samples.  angle, if not null, receives each keypoint's angle. */
+int bo_surf_descriptors(const uint8_t* img, size_t pitch, int width, int height, const float* kp,
+                        int ld, int n, int upright, float* desc, int desc_ld, double* angle);
 /* Brute-force 2-nearest-neighbour matching by L2 distance (cv::cuda BFMatcher knnMatch with
  * k = 2, bimp.cpp:188-205): for each query row, the train rows with the smallest and second
  * smallest distance sqrt(sum (q - t)^2), ties to the lower train index; sums in FP64.

@@ -74,6 +74,9 @@ def lib():
                                         C.c_int]
         L.bo_resize_linear.argtypes = [P, C.c_size_t, C.c_int, C.c_int, C.c_int, P, C.c_size_t,
                                        C.c_int, C.c_int, C.c_double, C.c_double]
+        L.bo_integral_image.argtypes = [P, C.c_size_t, C.c_int, C.c_int, P]
+        L.bo_surf_descriptors.argtypes = [P, C.c_size_t, C.c_int, C.c_int, P, C.c_int, C.c_int,
+                                          C.c_int, P, C.c_int, P]
         L.bo_knn2.argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P, P]
         _lib = L
     return _lib
@@ -107,6 +110,27 @@ def resize_linear(img, dw, dh, fx, fy):
     out = np.zeros((dh, dw) if cn == 1 else (dh, dw, cn), np.uint8)
     assert lib().bo_resize_linear(ptr(img), w * cn, w, h, cn, ptr(out), dw * cn, dw, dh, fx, fy) == 0
     return out
+
+
+def integral_image(img):
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    out = np.zeros((h + 1, w + 1), np.int64)
+    assert lib().bo_integral_image(ptr(img), w, w, h, ptr(out)) == 0
+    return out
+
+
+def surf_descriptors(img, kp_matrix, upright=False):
+    """(n, 64) SURF descriptors and (n,) angles of the keypoints of a 7 x n matrix."""
+    img = np.ascontiguousarray(img, np.uint8)
+    mat = np.ascontiguousarray(kp_matrix, np.float32)
+    h, w = img.shape
+    n = mat.shape[1]
+    desc = np.zeros((n, 64), np.float32)
+    ang = np.zeros(n, np.float64)
+    assert lib().bo_surf_descriptors(ptr(img), w, w, h, ptr(mat), n, n, int(upright), ptr(desc),
+                                     64, ptr(ang)) == 0
+    return desc, ang
 
 
 def knn2(query, train):

@@ -180,3 +180,61 @@ def test_resize_close_to_float_bilinear():
     bot = im[y1][:, x0] * (1 - fx) + im[y1][:, x1] * fx
     ref = top * (1 - fy)[:, None] + bot * fy[:, None]
     assert np.abs(got - ref).max() <= 1.0
+
+
+# ------------------------------------------------------------------ SURF descriptors
+def test_integral_image_and_haar_boxes():
+    rng = np.random.default_rng(4)
+    img = rng.integers(0, 256, (37, 53), dtype=np.uint8)
+    ii = O.integral_image(img)
+    assert np.array_equal(ii[1:, 1:], img.astype(np.int64).cumsum(0).cumsum(1))
+    assert np.all(ii[0] == 0) and np.all(ii[:, 0] == 0)
+
+
+def _edge(h, w, vertical):
+    img = np.full((h, w), 40, np.uint8)
+    if vertical:
+        img[:, w // 2:] = 200   # dark left, bright right
+    else:
+        img[h // 2:, :] = 200   # dark top, bright bottom
+    return img
+
+
+def test_surf_step_edges():
+    h, w = 120, 120
+    for vertical in (True, False):
+        img = _edge(h, w, vertical)
+        kp = _kp([(60.0, 60.0, 16.0)])
+        d, a = O.surf_descriptors(img, kp)
+        # every Haar response points across the edge: +x (angle 0) or +y (angle pi/2)
+        assert a[0] == (0.0 if vertical else np.pi / 2), a
+        assert abs(np.linalg.norm(d[0].astype(np.float64)) - 1) < 1e-6
+        du, _ = O.surf_descriptors(img, kp, upright=True)
+        f = du[0].reshape(16, 4)
+        if vertical:
+            assert np.all(f[:, 1] == 0) and np.all(f[:, 3] == 0)      # no dy
+            assert np.all(f[:, 0] >= 0) and np.array_equal(f[:, 0], f[:, 2])
+        else:
+            assert np.all(f[:, 0] == 0) and np.all(f[:, 2] == 0)      # no dx
+            assert np.all(f[:, 1] >= 0) and np.array_equal(f[:, 1], f[:, 3])
+        # in its own frame the oriented descriptor of either edge puts the response on the
+        # first axis: the same descriptor for both edges
+        fo = d[0].reshape(16, 4)
+        assert np.allclose(fo[:, 1], 0, atol=1e-6) and np.allclose(fo[:, 3], 0, atol=1e-6)
+        if vertical:
+            ref = fo.copy()
+        else:
+            assert np.allclose(fo, ref, atol=1e-6)
+
+
+def test_surf_contrast_invariance_and_norm():
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, 128, (90, 110), dtype=np.uint8)
+    pts = [(rng.uniform(5, 105), rng.uniform(5, 85), rng.choice([4.0, 5.66, 8.0, 11.3, 16.0]))
+           for _ in range(40)]
+    kp = _kp(pts)
+    d1, a1 = O.surf_descriptors(img, kp)
+    d2, a2 = O.surf_descriptors(img * 2, kp)
+    assert np.allclose(a1, a2, atol=1e-12)
+    assert np.allclose(d1, d2, atol=1e-6)
+    assert np.allclose(np.linalg.norm(d1.astype(np.float64), axis=1), 1, atol=1e-6)
