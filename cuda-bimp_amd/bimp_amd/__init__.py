@@ -83,7 +83,7 @@ EXPORTED = [
     "bimp_keypoints_cpu_mode", "bimp_time_detect_frames", "bimp_get_level",
     "bimp_color_features", "bimp_match_knn2", "bimp_ratio_matches",
     "bimp_min_size_shape", "bimp_resize_linear_u8", "bimp_get_gpu_image", "bimp_map_view",
-    "bimp_memcpy_h2d_2d",
+    "bimp_memcpy_h2d_2d", "bimp_detect_u8_noalloc",
 ]
 
 
@@ -148,6 +148,7 @@ def _load():
         "bimp_resize_linear_u8": (I, [P, SZ, I, I, I, P, SZ, I, I, C.c_double, C.c_double, P]),
         "bimp_get_gpu_image": (I, [P, SZ, I, I, I, I, P, SZ, P]),
         "bimp_memcpy_h2d_2d": (I, [P, SZ, P, SZ, SZ, SZ]),
+        "bimp_detect_u8_noalloc": (I, [P, P, SZ, I, P, I, C.POINTER(I), P]),
         "bimp_map_view": (I, [P, I, I, I, C.POINTER(P), C.POINTER(SZ), C.POINTER(I), C.POINTER(I)]),
         "bimp_match_knn2": (I, [P, I, I, P, I, I, I, P, P, P]),
         "bimp_ratio_matches": (I, [P, P, I, C.c_double, P, I, C.POINTER(I)]),
@@ -319,6 +320,21 @@ class CudaStuff:
         n = C.c_int(0)
         self._chk(lib.bimp_detect_u8(self.ctx, _ptr(img), self.width, 0, _ptr(out), cap, 0,
                                      C.byref(n), stream))
+        return out[:min(n.value, cap)].copy(), n.value
+
+    def detect_noalloc(self, img, cap=KP_MAX_POINTS, stream=None, pitch=None):
+        """bimp_detect_u8_noalloc: a host frame, or a DeviceBuffer frame with its pitch."""
+        out = np.zeros(max(cap, 1), KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        if isinstance(img, DeviceBuffer):
+            st = lib.bimp_detect_u8_noalloc(self.ctx, img.ptr, pitch or self.width, 1, _ptr(out),
+                                            cap, C.byref(n), stream)
+        else:
+            img = np.ascontiguousarray(img, np.uint8)
+            assert img.shape == (self.height, self.width)
+            st = lib.bimp_detect_u8_noalloc(self.ctx, _ptr(img), self.width, 0, _ptr(out), cap,
+                                            C.byref(n), stream)
+        self._chk(st)
         return out[:min(n.value, cap)].copy(), n.value
 
     def detect_matrix(self, img, cap=KP_MAX_POINTS, stream=None):

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -142,6 +143,9 @@ constexpr size_t kTBudgetBytes = 160u << 20;
 // so the lighter ones fill the tail of the heavier ones.
 constexpr int kDirectLaunchScales = 8;
 
+// Pyramid levels a context can hold (a frame side would have to exceed 2^31 to need more).
+constexpr int kMaxLevels = 32;
+
 int inverse_batch(size_t tstride_elems) {
     const size_t per = tstride_elems * sizeof(float2);
     return (int)std::max<size_t>(1, std::min<size_t>(bimpk::kMaxBatch, kTBudgetBytes / per));
@@ -195,8 +199,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         HIPCHK(c, hipStreamWaitEvent(c->side, c->evFork, 0));
     }
     const hipStream_t pq = useSide ? c->side : s;
-    std::vector<const uint8_t*> lv(c->maxLevel + 1);
-    std::vector<size_t> lp(c->maxLevel + 1);
+    const uint8_t* lv[kMaxLevels] = {};
+    size_t lp[kMaxLevels] = {};
     lv[0] = src0;
     lp[0] = pitch0;
     for (int l = 1; l <= c->maxLevel; ++l) {
@@ -398,10 +402,12 @@ int ensure_kp_buf(bimp_ctx* c, size_t bytes) {
 
 // Runs NMS into `out` (host or device) and reads the count back.
 int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_device, int* count,
-                  hipStream_t s) {
+                  hipStream_t s, void* staging = nullptr) {
     const size_t elem = matrix_ld > 0 ? 0 : sizeof(bimp_keypoint);
     void* target = out;
-    if (!out_on_device && cap > 0) {
+    if (!out_on_device && cap > 0 && staging) {
+        target = staging;
+    } else if (!out_on_device && cap > 0) {
         const size_t bytes = matrix_ld > 0 ? (size_t)BIMP_KP_VECTOR_LENGTH * matrix_ld * 4
                                            : (size_t)cap * elem;
         int st = ensure_kp_buf(c, bytes);
@@ -432,6 +438,39 @@ int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_dev
     return BIMP_OK;
 }
 
+}  // namespace
+
+// ---- keypoint staging of bimp_detect_u8_noalloc: no device allocation per call
+namespace {
+constexpr int kNoAllocMaxKeypoints = 65536;  // 65536 records x 28 B = 1.75 MiB per device
+}  // namespace
+
+// One copy per device, held until the process ends.  Every bimp_detect_u8_noalloc call on a
+// device shares it: calls that overlap on one device race.  Nothing in it needs clearing: a
+// call reads back only the records its own keypoint pass wrote.
+static __device__ __align__(256) bimp_keypoint scratch[kNoAllocMaxKeypoints];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<bimp_keypoint*> g_scratch;  // per device: the array's device address, or null
+
+// Device address of the current device's copy of `scratch`, looked up once per device with
+// that device current.  In host code the array's name is not a device address.
+hipError_t noalloc_staging(bimp_keypoint** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch.size() <= dev) g_scratch.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch[(size_t)dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[(size_t)dev] = static_cast<bimp_keypoint*>(p);
+    }
+    *out = g_scratch[(size_t)dev];
+    return hipSuccess;
+}
 }  // namespace
 
 int bimpi::run_frame(bimp_ctx* c, const uint8_t* d_img, size_t pitch, void* out, int cap,
@@ -545,6 +584,7 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bail(BIMP_E_DEVICE, "no HIP device");
     // pyramid levels
     for (auto& s : c->sc) c->maxLevel = std::max(c->maxLevel, s.g.level);
+    if (c->maxLevel >= kMaxLevels) return bail(BIMP_E_ARG, "too many pyramid levels");
     c->levelW.resize(c->maxLevel + 1);
     c->levelH.resize(c->maxLevel + 1);
     c->level.assign(c->maxLevel + 1, nullptr);
@@ -743,6 +783,8 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     if (hipMemcpy(c->nmsTable, tab.data(), tab.size() * sizeof(bimpk::NmsScale),
                   hipMemcpyHostToDevice) != hipSuccess)
         return bail(BIMP_E_DEVICE, "nms table upload");
+    if (noalloc_staging(&c->noallocStaging) != hipSuccess)
+        return bail(BIMP_E_DEVICE, "no-allocation staging lookup");
     (void)st;
     *out = c;
     return BIMP_OK;
@@ -811,6 +853,28 @@ int bimp_detect_u8(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_dev
     int st = run_filter(c, img, pitch, img_on_device, true, s);
     if (st) return st;
     return keypoints_out(c, out, cap, 0, out_on_device, count, s);
+}
+
+int bimp_detect_u8_noalloc(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_device,
+                           bimp_keypoint* out, int cap, int* count, bimp_stream_t stream) {
+    if (!c || !img || pitch < (size_t)c->width || cap < 0 || (cap > 0 && !out) || !count)
+        return fail(c, BIMP_E_ARG, "bad arguments");
+    if (cap > kNoAllocMaxKeypoints)
+        return fail(c, BIMP_E_ARG, "cap " + std::to_string(cap) +
+                                       " exceeds the no-allocation staging of " +
+                                       std::to_string(kNoAllocMaxKeypoints) + " keypoints");
+    if (c->opt.flags & BIMP_FLAG_PROFILE)
+        return fail(c, BIMP_E_UNSUPPORTED, "profiling creates events: clear BIMP_FLAG_PROFILE");
+    if (!img_on_device && pitch != (size_t)c->width && !c->hostIn)
+        return fail(c, BIMP_E_UNSUPPORTED,
+                    "a pitched host frame needs pinned staging: pass a device frame or a "
+                    "contiguous host frame");
+    if (!c->noallocStaging) return fail(c, BIMP_E_DEVICE, "no staging on this device");
+    hipStream_t s = (hipStream_t)stream;
+    c->lastStream = s;
+    int st = run_filter(c, img, pitch, img_on_device, true, s);
+    if (st) return st;
+    return keypoints_out(c, out, cap, 0, 0, count, s, c->noallocStaging);
 }
 
 int bimp_detect_u8_matrix(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_device,

@@ -106,6 +106,8 @@ struct bimp_ctx {
     float2* R2 = nullptr;
     float2* T2 = nullptr;
     size_t tCap2 = 0;
+    // this device's copy of the static keypoint staging of bimp_detect_u8_noalloc
+    bimp_keypoint* noallocStaging = nullptr;
 };
 
 namespace bimpi {

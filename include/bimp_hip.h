@@ -150,6 +150,16 @@ int bimp_detect_u8(bimp_ctx* ctx, const uint8_t* img, size_t pitch, int img_on_d
                    bimp_keypoint* out, int cap, int out_on_device, int* count,
                    bimp_stream_t stream);
 
+/* As bimp_detect_u8 with the keypoints delivered to host memory, without any device
+ * allocation: no counterpart in the reference.  For callers that cannot allocate (serving
+ * loops, code running between captured graphs, harnesses that forbid hipMalloc): the keypoint
+ * records are staged in a static device array of 65536 records (1.75 MiB, one per device, held
+ * until the process ends) instead of the context's lazily allocated buffer.  cap above 65536
+ * returns BIMP_E_ARG; a pitched host frame (which needs pinned staging the context allocates
+ * on first use) and a context with BIMP_FLAG_PROFILE return BIMP_E_UNSUPPORTED.  Calls on one
+ * device share the staging: calls that overlap on one device race. */
+int bimp_detect_u8_noalloc(bimp_ctx* ctx, const uint8_t* img, size_t pitch, int img_on_device,
+                           bimp_keypoint* out, int cap, int* count, bimp_stream_t stream);
 /* Same frame with the 7-row SURF_CUDA keypoint matrix the reference returns from
  * getKeypoints (cuda_bimp.cu:129-140): rows X, Y, LAPLACIAN (int bits), OCTAVE (int bits),
  * SIZE, ANGLE, HESSIAN, each of leading dimension ld >= cap floats. */
