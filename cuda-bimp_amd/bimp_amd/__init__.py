@@ -83,7 +83,8 @@ EXPORTED = [
     "bimp_keypoints_cpu_mode", "bimp_time_detect_frames", "bimp_get_level",
     "bimp_color_features", "bimp_match_knn2", "bimp_ratio_matches",
     "bimp_min_size_shape", "bimp_resize_linear_u8", "bimp_get_gpu_image", "bimp_map_view",
-    "bimp_memcpy_h2d_2d", "bimp_detect_u8_noalloc",
+    "bimp_memcpy_h2d_2d", "bimp_detect_u8_noalloc", "bimp_integral_image",
+    "bimp_surf_descriptors",
 ]
 
 
@@ -149,6 +150,8 @@ def _load():
         "bimp_get_gpu_image": (I, [P, SZ, I, I, I, I, P, SZ, P]),
         "bimp_memcpy_h2d_2d": (I, [P, SZ, P, SZ, SZ, SZ]),
         "bimp_detect_u8_noalloc": (I, [P, P, SZ, I, P, I, C.POINTER(I), P]),
+        "bimp_integral_image": (I, [P, SZ, I, I, P, P]),
+        "bimp_surf_descriptors": (I, [P, I, I, P, I, I, I, P, I, P, P]),
         "bimp_map_view": (I, [P, I, I, I, C.POINTER(P), C.POINTER(SZ), C.POINTER(I), C.POINTER(I)]),
         "bimp_match_knn2": (I, [P, I, I, P, I, I, I, P, P, P]),
         "bimp_ratio_matches": (I, [P, P, I, C.c_double, P, I, C.POINTER(I)]),
@@ -481,6 +484,29 @@ def color_features(kp_matrix, bgr):
                                    None))
     dd.download(out)
     return out
+
+
+def surf_descriptors(gray, kp_matrix, upright=False):
+    """SURF descriptors (n x 64) and angles (n,) of the keypoints of a 7-row matrix over a gray
+    uint8 frame, on the device (integral image, then one wavefront per keypoint)."""
+    gray = np.ascontiguousarray(gray, np.uint8)
+    mat = np.ascontiguousarray(kp_matrix, np.float32)
+    h, w = gray.shape
+    n = mat.shape[1]
+    desc = np.zeros((n, 64), np.float32)
+    ang = np.zeros(n, np.float32)
+    if n == 0:
+        return desc, ang
+    dg, di = DeviceBuffer(gray.nbytes), DeviceBuffer((w + 1) * (h + 1) * 4)
+    dk, dd, da = DeviceBuffer(mat.nbytes), DeviceBuffer(desc.nbytes), DeviceBuffer(ang.nbytes)
+    dg.upload(gray)
+    dk.upload(mat)
+    _check(lib.bimp_integral_image(dg.ptr, w, w, h, di.ptr, None))
+    _check(lib.bimp_surf_descriptors(di.ptr, w, h, dk.ptr, n, n, 1 if upright else 0, dd.ptr, 64,
+                                     da.ptr, None))
+    dd.download(desc)
+    da.download(ang)
+    return desc, ang
 
 
 def match_knn2(query, train):

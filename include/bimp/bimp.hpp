@@ -277,21 +277,55 @@ public:
         return out;
     }
 
-    // The colour histogram part of getDescriptors (bimp.cpp:32-59): BIMP_COLOR_DIMENSIONS
-    // floats per keypoint of a colour frame, computed on the device and downloaded.
-    std::vector<float> getColorFeaturesAndDownload() {
+    // getDescriptors (bimp.cpp:32-59): one BIMP_DIMENSIONS-float row per keypoint on the
+    // device: 64 SURF floats (bimp_surf_descriptors), zeros up to BIMP_SURF_DIMENSIONS, then
+    // the colour histogram (zeros for a gray frame).  The reference leaves the columns between
+    // its 64 SURF floats and the colour part unwritten.
+    struct DeviceDescriptors {
+        const float* d_desc = nullptr;
+        int ld = BIMP_DIMENSIONS;
+        int rows = 0;
+    };
+    const DeviceDescriptors& getDescriptors() {
         refresh();
-        if (!has_color_) check(BIMP_E_ARG);
+        if (!has_changed_descriptors_) return device_desc_;
         const int n = device_kp_.count;
-        std::vector<float> out((size_t)n * BIMP_COLOR_DIMENSIONS);
-        if (n == 0) return out;
-        desc_.reset(out.size() * sizeof(float));
-        check(bimp_color_features(device_kp_.d_matrix, device_kp_.ld, n, color_.get(),
-                                  (size_t)frame_w_ * 3, frame_w_, frame_h_, desc_.get<float>(),
-                                  BIMP_COLOR_DIMENSIONS, nullptr));
-        check(bimp_stream_sync(nullptr));
-        check(bimp_memcpy_d2h(out.data(), desc_.get(), out.size() * sizeof(float)));
+        desc_.reset((size_t)(n > 0 ? n : 1) * BIMP_DIMENSIONS * sizeof(float));
+        std::vector<float> zero((size_t)(n > 0 ? n : 1) * BIMP_DIMENSIONS, 0.f);
+        check(bimp_memcpy_h2d(desc_.get(), zero.data(), zero.size() * sizeof(float)));
+        if (n > 0) {
+            surf_descriptors_into(desc_.get<float>(), BIMP_DIMENSIONS);
+            if (has_color_)
+                check(bimp_color_features(device_kp_.d_matrix, device_kp_.ld, n, color_.get(),
+                                          (size_t)frame_w_ * 3, frame_w_, frame_h_,
+                                          desc_.get<float>() + BIMP_SURF_DIMENSIONS,
+                                          BIMP_DIMENSIONS, nullptr));
+            check(bimp_stream_sync(nullptr));
+        }
+        device_desc_ = DeviceDescriptors{desc_.get<float>(), BIMP_DIMENSIONS, n};
+        has_changed_descriptors_ = false;
+        return device_desc_;
+    }
+    std::vector<float> getDescriptorsAndDownload() {
+        const DeviceDescriptors& d = getDescriptors();
+        std::vector<float> out((size_t)d.rows * BIMP_DIMENSIONS);
+        if (!out.empty())
+            check(bimp_memcpy_d2h(out.data(), d.d_desc, out.size() * sizeof(float)));
         return out;
+    }
+    // The 64 SURF floats of each keypoint on the device (what matchContexts compares).
+    const DeviceDescriptors& getSurfDescriptors() {
+        refresh();
+        if (!has_changed_surf_) return device_surf_;
+        const int n = device_kp_.count;
+        surf_.reset((size_t)(n > 0 ? n : 1) * 64 * sizeof(float));
+        if (n > 0) {
+            surf_descriptors_into(surf_.get<float>(), 64);
+            check(bimp_stream_sync(nullptr));
+        }
+        device_surf_ = DeviceDescriptors{surf_.get<float>(), 64, n};
+        has_changed_surf_ = false;
+        return device_surf_;
     }
 
     // Context::setCPUMode (bimp.hpp:43-44): keypoints from the reference's CPU extraction rules
@@ -354,6 +388,16 @@ private:
         device_kp_.ld = cap;
         device_kp_.count = (int)keypoints_.size();
         has_changed_keypoints = false;
+        has_changed_descriptors_ = true;
+        has_changed_surf_ = true;
+    }
+    void surf_descriptors_into(float* d_out, int ld) {
+        integral_.reset((size_t)(frame_w_ + 1) * (frame_h_ + 1) * sizeof(unsigned));
+        check(bimp_integral_image(d_image_, image_pitch_, frame_w_, frame_h_,
+                                  integral_.get<unsigned>(), nullptr));
+        check(bimp_surf_descriptors(integral_.get<unsigned>(), frame_w_, frame_h_,
+                                    device_kp_.d_matrix, device_kp_.ld, device_kp_.count, 0,
+                                    d_out, ld, nullptr, nullptr));
     }
     static void warn(int count) {
         std::fprintf(stderr, "Warning: Keypoints (%d) exceed current keypoint maximum (%d).\n",
@@ -362,7 +406,9 @@ private:
     bool cpu_mode_ = false;
     std::vector<float> lambdas_;
     int width_ = 0, height_ = 0;
-    DeviceBytes raw_, image_, color_, matrix_d_, desc_;
+    DeviceBytes raw_, image_, color_, matrix_d_, desc_, surf_, integral_;
+    DeviceDescriptors device_desc_, device_surf_;
+    bool has_changed_descriptors_ = true, has_changed_surf_ = true;
     const uint8_t* d_image_ = nullptr;
     size_t image_pitch_ = 0;
     int frame_w_ = 0, frame_h_ = 0;
@@ -370,6 +416,29 @@ private:
     std::vector<KeyPoint> keypoints_;
     DeviceKeypoints device_kp_;
 };
+
+// matchContexts / matchContextsAndDownload / downloadMatches (bimp.cpp:131-240): the SURF
+// descriptors of both contexts' keypoints, brute-force 2-nearest-neighbour matching on the
+// device, then the ratio test dist1 < 0.6 * dist2 on the host.
+inline std::vector<bimp_match> matchContextsAndDownload(Context& c1, Context& c2) {
+    const Context::DeviceDescriptors& q = c1.getSurfDescriptors();
+    const Context::DeviceDescriptors& t = c2.getSurfDescriptors();
+    std::vector<bimp_match> out;
+    if (q.rows == 0) return out;
+    DeviceBytes idx((size_t)q.rows * 2 * sizeof(int)), dist((size_t)q.rows * 2 * sizeof(float));
+    check(bimp_match_knn2(q.d_desc, q.rows, q.ld, t.d_desc, t.rows, t.ld, 64, idx.get<int>(),
+                          dist.get<float>(), nullptr));
+    check(bimp_stream_sync(nullptr));
+    std::vector<int> hi((size_t)q.rows * 2);
+    std::vector<float> hd((size_t)q.rows * 2);
+    check(bimp_memcpy_d2h(hi.data(), idx.get(), hi.size() * sizeof(int)));
+    check(bimp_memcpy_d2h(hd.data(), dist.get(), hd.size() * sizeof(float)));
+    out.resize((size_t)q.rows);
+    int n = 0;
+    check(bimp_ratio_matches(hi.data(), hd.data(), q.rows, 0.6, out.data(), q.rows, &n));
+    out.resize((size_t)n);
+    return out;
+}
 
 }  // namespace bimp
 

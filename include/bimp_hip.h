@@ -315,6 +315,21 @@ typedef struct {
  * synchronisation. */
 int bimp_color_features(const float* d_kp, int ld, int n, const uint8_t* d_bgr, size_t bgr_pitch,
                         int width, int height, float* d_desc, int desc_ld, bimp_stream_t stream);
+/* Integral image of a device 8-bit frame: (height+1) x (width+1) uint32 sums, first row and
+ * column 0 (SURF's box filters; width*height*255 must stay below 2^31).  No synchronisation. */
+int bimp_integral_image(const uint8_t* d_img, size_t pitch, int width, int height,
+                        unsigned* d_integral, bimp_stream_t stream);
+/* SURF descriptors of given keypoints (the reference runs cv::cuda::SURF_CUDA with
+ * useProvidedKeypoints on the BIMP keypoints, bimp.cpp:32-59, :188-199), restated from the
+ * published method (Bay et al. 2008; OpenCV's code is not followed): scale 1.2*size/9, the
+ * orientation from Haar responses of a disc of radius 6s (72 windows of pi/3; upright != 0 keeps
+ * angle 0), 4 x 4 subregions of 5 x 5 rotated samples summing dx, dy, |dx|, |dy|, unit length.
+ * 64 floats per keypoint at d_desc + i*desc_ld; d_angle (may be NULL) gets the angles in
+ * radians.  Keypoints are a device 7-row SURF_CUDA matrix (leading dimension ld).  No
+ * synchronisation. */
+int bimp_surf_descriptors(const unsigned* d_integral, int width, int height, const float* d_kp,
+                          int ld, int n, int upright, float* d_desc, int desc_ld, float* d_angle,
+                          bimp_stream_t stream);
 /* Brute-force 2-nearest-neighbour matching by L2 distance (cv::cuda::DescriptorMatcher
  * createBFMatcher + knnMatchAsync with k = 2, bimp.cpp:202-204): for each of nq device query
  * rows (q_ld floats apart) the two train rows (t_ld apart) with the smallest distances,

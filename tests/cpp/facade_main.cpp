@@ -6,7 +6,8 @@
 //   writes: int32 prepared frame width and height; int32 count, count keypoint records, the
 //   SURF_CUDA matrix (7 x count, from the device); the response views' scale count and, per
 //   scale, the D map read through its device view; int32 count and records of the CPU
-//   keypoint mode; for BGR frames, count x 24 colour-histogram floats (device keypoints).
+//   keypoint mode; count x 152 descriptor floats (SURF, zeros, colour histogram); the
+//   matches of the context against itself.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -64,10 +65,14 @@ int main(int argc, char** argv) {
         put_int(nc);
         out.write(reinterpret_cast<const char*>(cpu.data()), (std::streamsize)(nc * sizeof(bimp::KeyPoint)));
         ctx.setCPUMode(false);
-        if (cn == 3) {
-            const std::vector<float> col = ctx.getColorFeaturesAndDownload();
-            out.write(reinterpret_cast<const char*>(col.data()), (std::streamsize)(col.size() * sizeof(float)));
-        }
+        // getDescriptors: count x BIMP_DIMENSIONS (SURF, zeros, colour histogram)
+        const std::vector<float> desc = ctx.getDescriptorsAndDownload();
+        if (desc.size() != (size_t)n * BIMP_DIMENSIONS) return 5;
+        out.write(reinterpret_cast<const char*>(desc.data()), (std::streamsize)(desc.size() * sizeof(float)));
+        // matchContexts against itself: every keypoint's nearest neighbour is itself
+        const std::vector<bimp_match> self = bimp::matchContextsAndDownload(ctx, ctx);
+        put_int((int)self.size());
+        out.write(reinterpret_cast<const char*>(self.data()), (std::streamsize)(self.size() * sizeof(bimp_match)));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

@@ -87,9 +87,19 @@ def test_context_facade_matches_c_abi(tmp_path, exe, w, h, cn, bank):
     want_cpu, mc = cs.keypoints_cpu_mode()
     assert nc == mc and nc > 0
     assert np.array_equal(bits(cpu), bits(want_cpu))
+    desc = r.take(n * 152, np.float32).reshape(n, 152)
+    surf, _ = B.surf_descriptors(prepared, B.keypoints_to_matrix(want))
+    assert np.array_equal(desc[:, :64].view(np.uint32), surf.view(np.uint32))
+    assert np.all(desc[:, 64:128] == 0)
     if cn == 3:
-        col = r.take(n * 24, np.float32).reshape(n, 24)
         color = B.get_gpu_image(img, grayscale=False)
         want_col = O.color_features(B.keypoints_to_matrix(want), color)
-        assert np.array_equal(col.view(np.uint32), want_col.view(np.uint32))
+        assert np.array_equal(desc[:, 128:].view(np.uint32), want_col.view(np.uint32))
+    else:
+        assert np.all(desc[:, 128:] == 0)
+    nm = r.int()
+    m = r.take(nm, B.MATCH_DTYPE)
+    # matched against itself: the nearest neighbour is the keypoint itself at distance 0 (the
+    # ratio test keeps it unless a second keypoint has the same descriptor)
+    assert nm > 0.9 * n and np.all(m["query_idx"] == m["train_idx"]) and np.all(m["distance"] == 0)
     assert r.o == len(r.raw)
