@@ -308,7 +308,7 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                 LaunchScope ls(c, q, F_INV_ROWS,
                                m * (8.0 * kNumOri * s0.Hc * g.FW + 4.0 * kNumOri * s0.Hc * s0.Wc),
                                g.FW);
-                HIPCHK(c, bimpk::launch_inv_rows(g.FW, sub, T, tstride, s0.Hc, s0.Wc, s0.ldc,
+                HIPCHK(c, bimpk::launch_inv_rows(g.FW, g.FH, sub, T, tstride, s0.Hc, s0.Wc, s0.ldc,
                                                  twW, q));
             }
             if (cells) {

@@ -301,61 +301,60 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
     }
 }
 
-// LDS floats one group's exchange region needs, and the address of element i in it.  After
-// the first stage lane j writes its R outputs at j*R + r: with R = 16 that stride would put 32
-// lanes on 2 banks.  That exchange XOR-swizzles the bank bits of each 32-word row with the row
-// index (a bijection within the row), which spreads those writes over all banks and keeps the
-// reads (32 consecutive elements of one row per access) conflict-free, at no extra LDS.
-// Later exchanges write runs of >= R consecutive lanes and keep the plain layout.
+// LDS floats one group's exchange region needs.
 template <int N>
 __host__ __device__ constexpr int fft_lds_floats() { return N; }
-template <int Ns, int R>
-__device__ __forceinline__ int lds_pad(int i) {
-    if constexpr (Ns == 1)
-        return (i & ~31) | ((i ^ (i >> 5)) & 31);
-    else
-        return i;
-}
 
 // The exchange moves real parts, then imaginary parts, through an LDS region of
-// fft_lds_floats<N>() floats: a register slot keeps its old imaginary part until the second
-// half has been written, so half the LDS of a complex exchange suffices.
+// fft_lds_floats<N>() floats: a register slot keeps its imaginary part until the second half
+// has been written, so half the LDS of a complex exchange suffices.
+//
+// After the first stage (Ns = 1) lane j holds the R consecutive outputs j*R .. j*R+R-1: they
+// leave as R/4 float4 writes (every first radix is 4, 8 or 16), so consecutive lanes' 16-byte
+// writes lie 4R bytes apart and one instruction replaces R scalar writes.  Later stages write
+// each element at (j/Ns)*Ns*R + j%Ns + r*Ns, whose r-dependent part is an immediate offset.
+// Reads are j + r*(N/R2) for the next radix R2: consecutive lanes, consecutive words.
 template <int N, int Ns, int R, int R2>
 __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, int lane) {
     using C = FFTCfg<N>;
 #pragma unroll
-    for (int m = 0; m < C::E / R; ++m) {
-        const int j = lane + C::TPV * m;
-        const int base = (j / Ns) * Ns * R + (j % Ns);
+    for (int half = 0; half < 2; ++half) {
+        if constexpr (Ns == 1) {
+            static_assert(R % 4 == 0, "first radix 4, 8 or 16");
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R>(base + r * Ns)] = v[m * R + r].x;
+            for (int m = 0; m < C::E / R; ++m) {
+                float4* dst = reinterpret_cast<float4*>(lds + (lane + C::TPV * m) * R);
+#pragma unroll
+                for (int q = 0; q < R / 4; ++q) {
+                    const float2* p = &v[m * R + 4 * q];
+                    dst[q] = half ? make_float4(p[0].y, p[1].y, p[2].y, p[3].y)
+                                  : make_float4(p[0].x, p[1].x, p[2].x, p[3].x);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < C::E / R; ++m) {
+                const int j = lane + C::TPV * m;
+                float* dst = lds + (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * Ns] = half ? v[m * R + r].y : v[m * R + r].x;
+            }
+        }
+        stage_sync();
+#pragma unroll
+        for (int m = 0; m < C::E / R2; ++m) {
+            const float* src = lds + lane + C::TPV * m;
+#pragma unroll
+            for (int r = 0; r < R2; ++r) {
+                const float x = src[r * (N / R2)];
+                if (half)
+                    v[m * R2 + r].y = x;
+                else
+                    v[m * R2 + r].x = x;
+            }
+        }
+        stage_sync();
     }
-    stage_sync();
-    float im[C::E];
-#pragma unroll
-    for (int e = 0; e < C::E; ++e) im[e] = v[e].y;
-#pragma unroll
-    for (int m = 0; m < C::E / R2; ++m) {
-        const int j = lane + C::TPV * m;
-#pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[lds_pad<Ns, R>(j + r * (N / R2))];
-    }
-    stage_sync();
-#pragma unroll
-    for (int m = 0; m < C::E / R; ++m) {
-        const int j = lane + C::TPV * m;
-        const int base = (j / Ns) * Ns * R + (j % Ns);
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R>(base + r * Ns)] = im[m * R + r];
-    }
-    stage_sync();
-#pragma unroll
-    for (int m = 0; m < C::E / R2; ++m) {
-        const int j = lane + C::TPV * m;
-#pragma unroll
-        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[lds_pad<Ns, R>(j + r * (N / R2))];
-    }
-    stage_sync();
 }
 
 template <int N, bool INV, int Ns, class L>

@@ -127,11 +127,21 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
     using C = FFTCfg<FH>;
     using CC = ColCfg<FH>;
     extern __shared__ float2 lds2[];
-    const int sidx = blockIdx.y;
+    // Workgroups are dealt round-robin to the 8 XCDs by linear id; pairs of adjacent column
+    // blocks go to one XCD, so the two halves of each 128-byte row segment of R are fetched
+    // into the same L2 (full groups of 16 blocks; the remainder keeps its order).
+    int sidx, blk;
+    {
+        const int nb = gridDim.x, total = nb * gridDim.y;
+        const int L = blockIdx.x + blockIdx.y * nb, full = total / 16 * 16;
+        const int p = L < full ? 16 * (L / 16) + 2 * (L % 8) + ((L / 8) & 1) : L;
+        sidx = p / nb;
+        blk = p % nb;
+    }
     const int kY = ss.kX[sidx];
     const float2* R = Rbase + (size_t)sidx * rstride;
     float2* DT = ss.DT[sidx];
-    const int kx0 = blockIdx.x * CC::CB;
+    const int kx0 = blk * CC::CB;
     {
         // a thread stages column b of the block, rows y0 + i*YS; eight loads go out before
         // their LDS stores
@@ -207,7 +217,11 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
     constexpr int CB = IC::CB;
     extern __shared__ float2 lds2[];
     const int half = FW / 2;
-    const int g = threadIdx.x / C::TPV, lane0 = threadIdx.x % C::TPV;
+    // with one transform per wavefront the group index is uniform per wavefront: kept
+    // scalar, so every column pointer is (smaller transforms put several groups in a wavefront)
+    const int g = C::TPV == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x / 64)
+                               : (int)(threadIdx.x / C::TPV);
+    const int lane0 = threadIdx.x % C::TPV;
     float* my = reinterpret_cast<float*>(lds2) + (size_t)g * fft_lds_floats<FH>();
     float4* twl = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds2) + IC::MAIN);
     if constexpr (IC::LDS_TW) {
@@ -241,12 +255,25 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
         // bank column: direct o <= 4 -> kx, o >= 5 -> FW-kx; partner o <= 4 -> FW-kx, else kx
         const int cg = ((o <= 4) != (partner != 0)) ? kx : (kx == 0 ? 0 : FW - kx);
         const float* gcol = G + ((size_t)og * FW + (ok ? cg : 0)) * FH;
+        // D * G_o: per-lane base pointers, element offsets in_index(e, 0) known at compile
+        // time; the partner pass reads G at row FH - ky, except ky = 0 (lane 0 of e = 0)
         float2 v[C::E];
+        const float2* dl = dcol + lane;
+        if (partner) {
+            const float* gl = gcol + (FH - lane);
 #pragma unroll
-        for (int e = 0; e < C::E; ++e) {
-            const int ky = in_index<FH>(e, lane);
-            const int kg = (partner && ky) ? FH - ky : ky;
-            v[e] = cscale(dcol[ky], gcol[kg]);
+            for (int e = 0; e < C::E; ++e) {
+                const int ce = in_index<FH>(e, 0);
+                const float gv = e == 0 ? gcol[lane ? FH - lane : 0] : gl[-ce];
+                v[e] = cscale(dl[ce], gv);
+            }
+        } else {
+            const float* gl = gcol + lane;
+#pragma unroll
+            for (int e = 0; e < C::E; ++e) {
+                const int ce = in_index<FH>(e, 0);
+                v[e] = cscale(dl[ce], gl[ce]);
+            }
         }
         if constexpr (IC::LDS_TW)
             fft<FH, true>(v, my, lane,
@@ -279,24 +306,26 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
             for (int m = 0; m < C::E / IC::RL; ++m)
                 lds2[slot * IC::SS + lane + C::TPV * m] = v[m * IC::RL + rb];
             __syncthreads();
+            // T is column-block-major: block base / CB holds rows 0..Hc-1 of its CB columns
+            // contiguously, so a unit's rows go out as one contiguous run
+            float2* const blk = To + (size_t)(base / CB) * Hc * CB;
             if constexpr (BATCH) {
-                const size_t rstep = (size_t)YS * FW;
 #pragma unroll 1
                 for (int i0 = 0; i0 < IT; i0 += 4) {
                     float2 val[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) val[j] = lds2[b * IC::SS + yy0 + (i0 + j) * YS];
-                    float2* row0 = To + (size_t)(ylo + yy0 + i0 * YS) * FW + base + b;
+                    float2* dst = blk + (size_t)(ylo + yy0 + i0 * YS) * CB + b;
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        if (wcol && yy0 + (i0 + j) * YS < rows) row0[(size_t)j * rstep] = val[j];
+                        if (wcol && yy0 + (i0 + j) * YS < rows) dst[(size_t)j * YS * CB] = val[j];
                 }
             } else {
                 for (int t = threadIdx.x; t < rows * CB; t += IC::THREADS) {
                     const int bt = t % CB, yy = t / CB;
                     const int kst = partner ? k0 + CB - bt : k0 + bt;  // stored column
                     const bool w = partner ? (kst < half) : (kst <= half);
-                    if (w) To[(size_t)(ylo + yy) * FW + base + bt] = lds2[bt * IC::SS + yy];
+                    if (w) blk[(size_t)(ylo + yy) * CB + bt] = lds2[bt * IC::SS + yy];
                 }
             }
         }
@@ -308,9 +337,30 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
 // Inverse row FFT of T_o row y and complex-cell magnitude sqrt(e^2 + o^2) for x < Wc
 // (complexResponse_kernel, cuda_bimp.cuh:165-180).  blockIdx.y = orientation,
 // blockIdx.z = scale of the batch.
+// Row y of T_o in the column-block-major layout the column pass writes (blocks of 2^tsh
+// columns, rows contiguous within a block): element x sits at
+// ((x >> tsh) * Hc + y) << tsh | (x & (2^tsh - 1)) from the orientation's base.
+template <int FW>
+__device__ __forceinline__ void load_t_row(float2 (&v)[FFTCfg<FW>::E], const float2* To, int Hc,
+                                           int y, int tsh, int lane) {
+    using C = FFTCfg<FW>;
+    if constexpr (C::TPV == 64) {
+        // in_index(e, lane) = lane + c_e with c_e a multiple of 64 >= the block width
+        const float2* p = To + ((((size_t)(lane >> tsh) * Hc + y) << tsh) | (lane & ((1 << tsh) - 1)));
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) v[e] = p[(size_t)in_index<FW>(e, 0) * Hc];
+    } else {
+#pragma unroll
+        for (int e = 0; e < C::E; ++e) {
+            const int x = in_index<FW>(e, lane);
+            v[e] = To[(((size_t)(x >> tsh) * Hc + y) << tsh) | (x & ((1 << tsh) - 1))];
+        }
+    }
+}
+
 template <int FW>
 __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __restrict__ Tbase,
-                                                 size_t tstride, int Hc, int Wc, int ldc,
+                                                 size_t tstride, int Hc, int Wc, int ldc, int tsh,
                                                  const float2* __restrict__ tw) {
     using C = FFTCfg<FW>;
     extern __shared__ float lds[];
@@ -318,10 +368,9 @@ __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __re
     const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
     const int y = blockIdx.x * RowCfg<FW>::VPB + g;
     const bool active = y < Hc;
-    const float2* row = Tbase + (size_t)sidx * tstride + ((size_t)o * Hc + (active ? y : 0)) * FW;
     float2 v[C::E];
-#pragma unroll
-    for (int e = 0; e < C::E; ++e) v[e] = row[in_index<FW>(e, lane)];
+    load_t_row<FW>(v, Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW, Hc, active ? y : 0,
+                   tsh, lane);
     fft<FW, true>(v, lds + (size_t)g * fft_lds_floats<FW>(), lane, tw);
     if (active) {
         float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
@@ -350,7 +399,7 @@ struct InvRowCfg {
 template <int FW>
 __global__ void __launch_bounds__(InvRowCfg<FW>::THREADS) k_inv_rows_p(
         ScaleSet ss, const float2* __restrict__ Tbase, size_t tstride, int Hc, int Wc, int ldc,
-        int units, const float2* __restrict__ tw) {
+        int tsh, int units, const float2* __restrict__ tw) {
     using C = FFTCfg<FW>;
     using RC = InvRowCfg<FW>;
     static_assert(C::TPV == 64, "one row per wavefront");
@@ -369,11 +418,9 @@ __global__ void __launch_bounds__(InvRowCfg<FW>::THREADS) k_inv_rows_p(
         asm volatile("" : "+v"(lane));
         const int y = rb * RC::WPB + w;
         const bool active = y < Hc;
-        const float2* row =
-            Tbase + (size_t)sidx * tstride + ((size_t)o * Hc + (active ? y : 0)) * FW;
         float2 v[C::E];
-#pragma unroll
-        for (int e = 0; e < C::E; ++e) v[e] = row[in_index<FW>(e, lane)];
+        load_t_row<FW>(v, Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW, Hc,
+                       active ? y : 0, tsh, lane);
         fft<FW, true>(v, my, lane, LdsTw{(const __attribute__((address_space(3))) float4*)twl});
         if (active) {
             float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
@@ -1186,8 +1233,13 @@ hipError_t launch_inv_cols(int FH, const ScaleSet& ss, int FW, int Hc, float2* T
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t tstride, int Hc,
-                           int Wc, int ldc, const float2* tw, hipStream_t s) {
+hipError_t launch_inv_rows(int FW, int FH, const ScaleSet& ss, const float2* T, size_t tstride,
+                           int Hc, int Wc, int ldc, const float2* tw, hipStream_t s) {
+    // the column pass's block width (columns per T block) for transforms of length FH
+    const int cb = col_batch(FH);
+    int tsh = 0;
+    while ((1 << tsh) < cb) ++tsh;
+    if (cb <= 0 || (1 << tsh) != cb || FW % cb) return hipErrorInvalidValue;
 #define X(N)                                                                                 \
     if (FW == N) {                                                                           \
         if constexpr (InvRowCfg<N>::PERSISTENT) {                                            \
@@ -1196,13 +1248,13 @@ hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t t
             const int grid = std::min(units, inv_rows_slots<N>());                           \
             if (grid <= 0) return hipErrorInvalidValue;                                      \
             k_inv_rows_p<N><<<grid, RC::THREADS, RC::LDS, s>>>(ss, T, tstride, Hc, Wc, ldc,  \
-                                                               units, tw);                   \
+                                                               tsh, units, tw);              \
             return hipGetLastError();                                                        \
         }                                                                                    \
         const int vpb = RowCfg<N>::VPB;                                                      \
         dim3 grid((Hc + vpb - 1) / vpb, kNumOri, ss.n);                                      \
         k_inv_rows<N><<<grid, RowCfg<N>::THREADS, vpb * fft_lds_floats<N>() * sizeof(float), s>>>( \
-            ss, T, tstride, Hc, Wc, ldc, tw);                                                \
+            ss, T, tstride, Hc, Wc, ldc, tsh, tw);                                           \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)

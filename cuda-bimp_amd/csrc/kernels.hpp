@@ -109,8 +109,8 @@ hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, size_t rstride, int
                            const ScaleSet& ss, int ncols, const float2* tw, hipStream_t s);
 hipError_t launch_inv_cols(int FH, const ScaleSet& ss, int FW, int Hc, float2* T,
                            size_t tstride, const float2* tw, hipStream_t s);
-hipError_t launch_inv_rows(int FW, const ScaleSet& ss, const float2* T, size_t tstride, int Hc,
-                           int Wc, int ldc, const float2* tw, hipStream_t s);
+hipError_t launch_inv_rows(int FW, int FH, const ScaleSet& ss, const float2* T, size_t tstride,
+                           int Hc, int Wc, int ldc, const float2* tw, hipStream_t s);
 hipError_t launch_cells(const float* C, int Hc, int ldc, int FH, int FW, int H, int W,
                         const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
                         hipStream_t s);
