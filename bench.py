@@ -267,12 +267,13 @@ def batch_form(B, stream, frames=64, seed=3, iters=3):
     return out
 
 
-def video_form(B, cs, frames=200, seq=16):
+def video_form(B, cs, frames=200, seq=16, graph=False):
     """Headless video loop (videoKeypoints, example_bimp.cpp:116-196, without capture, resize
     or rendering): each frame is a host uint8 array that is uploaded, filtered and reduced to
     keypoints, which come back to the host before the next frame, as the reference's loop
     does.  The sequence is 16 synthetic 1920x1080 frames (300 shapes, seeds 3000 + i), cycled."""
     seqf = [B.synth_frame(W, H, NSHAPES, 3000 + i) for i in range(seq)]
+    cs.set_flags(B.FLAG_GRAPH if graph else 0)
     for f in seqf[:2]:
         cs.detect(f)
     t0 = time.perf_counter()
@@ -281,7 +282,9 @@ def video_form(B, cs, frames=200, seq=16):
         _, n = cs.detect(seqf[i % seq])
         total += n
     dt = time.perf_counter() - t0
+    cs.set_flags(0)
     return {"frames": frames, "sequence": f"{seq} synthetic frames cycled",
+            "launch": "one hipGraph per frame" if graph else "eager launches",
             "fps_host_to_host": round(frames / dt, 1), "ms_per_frame": round(dt / frames * 1e3, 4),
             "keypoints_per_frame": round(total / frames, 1),
             "note": "pageable host frame in, keypoint records out to host, one frame at a time"}
@@ -559,9 +562,19 @@ def main():
             best = max(ok, key=lambda n: forms[n]["fps"])
             out["precision_form_best"] = {"form": best, "fps": forms[best]["fps"],
                                           "speedup_vs_plain": forms[best]["speedup_vs_plain"]}
+    # the same device-resident frames replayed as one hipGraph each (BIMP_FLAG_GRAPH)
+    cs.set_flags(base_flags | B.FLAG_GRAPH)
+    cs.time_detect_frames(d_img, W, d_out, cap, max(args.warmup, 1), stream)
+    per_g = cs.time_detect_frames(d_img, W, d_out, cap, args.steps, stream)
+    cs.set_flags(base_flags)
+    out["graph_replay"] = {"fps": round(1000.0 * args.steps / float(per_g.astype(np.float64).sum()), 2),
+                           "median_ms": round(float(np.median(per_g)), 4),
+                           "p99_ms": round(float(np.percentile(per_g, 99)), 4)}
     if args.form == "plain":
         out["video_form"] = video_form(B, cs)
-        print(f"video form: {out['video_form']['fps_host_to_host']} frames/s host to host",
+        out["video_form_graph"] = video_form(B, cs, graph=True)
+        print(f"video form: {out['video_form']['fps_host_to_host']} frames/s host to host "
+              f"({out['video_form_graph']['fps_host_to_host']} with a graph per frame)",
               file=sys.stderr, flush=True)
     if not args.no_batch and args.form == "plain":
         out["batch_form"] = batch_form_child()

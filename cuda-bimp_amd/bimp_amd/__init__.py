@@ -25,6 +25,7 @@ OK, W_OVERFLOW, E_ARG, E_ALLOC, E_DEVICE, E_UNSUPPORTED = 0, 1, -1, -2, -3, -4
 MAP_DOUBLE, MAP_SINGLE, MAP_COMPLEX = 0, 1, 2
 FLAG_PROFILE = 1
 FLAG_SINGLE_STREAM = 2
+FLAG_GRAPH = 4
 PRECISION_FP32, PRECISION_BF16, PRECISION_FP16 = 0, 1, 2
 DIRECT_MAX_KERNEL = 55
 
@@ -322,6 +323,14 @@ class CudaStuff:
         out = np.zeros(max(cap, 1), KEYPOINT_DTYPE)
         n = C.c_int(0)
         self._chk(lib.bimp_detect_u8(self.ctx, _ptr(img), self.width, 0, _ptr(out), cap, 0,
+                                     C.byref(n), stream))
+        return out[:min(n.value, cap)].copy(), n.value
+
+    def detect_device(self, d_img, pitch, cap=KP_MAX_POINTS, stream=None):
+        """bimp_detect_u8 on a device frame (DeviceBuffer, row pitch in bytes)."""
+        out = np.zeros(max(cap, 1), KEYPOINT_DTYPE)
+        n = C.c_int(0)
+        self._chk(lib.bimp_detect_u8(self.ctx, d_img.ptr, int(pitch), 1, _ptr(out), cap, 0,
                                      C.byref(n), stream))
         return out[:min(n.value, cap)].copy(), n.value
 

@@ -109,6 +109,8 @@ void free_all(bimp_ctx* c) {
     (void)hipFree(c->T);
     (void)hipFree(c->R2);
     (void)hipFree(c->T2);
+    if (c->frameExec) (void)hipGraphExecDestroy(c->frameExec);
+    if (c->capStream) (void)hipStreamDestroy(c->capStream);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->evFork) (void)hipEventDestroy(c->evFork);
     if (c->evJoin) (void)hipEventDestroy(c->evJoin);
@@ -156,34 +158,48 @@ int inverse_batch(size_t tstride_elems) {
 // end-stop/inhibition stencil.  Direct-form scales replace the transforms by one MFMA
 // convolution launch.  Running the consumers right after their producers keeps the
 // intermediate T and the complex-cell maps resident in the 256 MB last-level cache.
-int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, bool cells,
-               hipStream_t s) {
+// A host frame into the context's level-0 buffer on s: contiguous frames with one copy, pitched
+// ones gathered into pinned staging first (a 2D copy from pageable memory is slow).
+int upload_frame(bimp_ctx* c, const uint8_t* img, size_t pitch, hipStream_t s) {
     const int W0 = c->width, H0 = c->height;
-    {
+    const size_t bytes = (size_t)W0 * H0;
+    if (pitch == (size_t)W0) {
+        HIPCHK(c, hipMemcpyAsync(c->level[0], img, bytes, hipMemcpyHostToDevice, s));
+        return BIMP_OK;
+    }
+    if (!c->hostIn) {
+        HIPCHK(c, hipHostMalloc((void**)&c->hostIn, bytes, hipHostMallocDefault));
+        HIPCHK(c, hipEventCreateWithFlags(&c->hostInDone, hipEventDisableTiming));
+    } else {
+        HIPCHK(c, hipEventSynchronize(c->hostInDone));
+    }
+    for (int y = 0; y < H0; ++y)
+        std::memcpy(c->hostIn + (size_t)y * W0, img + (size_t)y * pitch, (size_t)W0);
+    HIPCHK(c, hipMemcpyAsync(c->level[0], c->hostIn, bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipEventRecord(c->hostInDone, s));
+    return BIMP_OK;
+}
+
+// Parts of a frame for graph capture (run_filter's `part`): the whole frame as issued eagerly,
+// or, captured one per graph: the per-frame reset, the work on levels above 0 (pyramid
+// included), the level-0 work, or both of those in order on one stream.
+enum FramePart { kPartAll = 0, kPartHead, kPartUpper, kPartLevel0, kPartBody };
+
+int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, bool cells,
+               hipStream_t s, int part = kPartAll) {
+    const int W0 = c->width, H0 = c->height;
+    if (part == kPartAll || part == kPartHead) {
         LaunchScope ls(c, s, F_INIT, 0);
         HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
+        c->maxValid = false;
+        if (part == kPartHead) return BIMP_OK;
     }
-    c->maxValid = false;
     const uint8_t* src0 = img;
     size_t pitch0 = pitch;
-    if (!on_device) {
+    if (!on_device && part == kPartAll) {
         LaunchScope ls(c, s, F_COPY_IN, 2.0 * W0 * H0);
-        const size_t bytes = (size_t)W0 * H0;
-        if (pitch == (size_t)W0) {
-            HIPCHK(c, hipMemcpyAsync(c->level[0], img, bytes, hipMemcpyHostToDevice, s));
-        } else {
-            // pitched host frame: rows into pinned staging, then one contiguous upload
-            if (!c->hostIn) {
-                HIPCHK(c, hipHostMalloc((void**)&c->hostIn, bytes, hipHostMallocDefault));
-                HIPCHK(c, hipEventCreateWithFlags(&c->hostInDone, hipEventDisableTiming));
-            } else {
-                HIPCHK(c, hipEventSynchronize(c->hostInDone));
-            }
-            for (int y = 0; y < H0; ++y)
-                std::memcpy(c->hostIn + (size_t)y * W0, img + (size_t)y * pitch, (size_t)W0);
-            HIPCHK(c, hipMemcpyAsync(c->level[0], c->hostIn, bytes, hipMemcpyHostToDevice, s));
-            HIPCHK(c, hipEventRecord(c->hostInDone, s));
-        }
+        const int st = upload_frame(c, img, pitch, s);
+        if (st) return st;
         src0 = c->level[0];
         pitch0 = W0;
     }
@@ -191,7 +207,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
     // the side stream, forked once the frame is on the device, except while s is being
     // captured into a graph (there it stays on s).  Level-0 work reads the frame only.
     hipStreamCaptureStatus capturing = hipStreamCaptureStatusActive;
-    const bool useSide = c->side && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM) &&
+    const bool useSide = part == kPartAll && c->side &&
+                         !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM) &&
                          hipStreamIsCapturing(s, &capturing) == hipSuccess &&
                          capturing == hipStreamCaptureStatusNone;
     if (useSide) {
@@ -204,6 +221,11 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
     lv[0] = src0;
     lp[0] = pitch0;
     for (int l = 1; l <= c->maxLevel; ++l) {
+        if (part == kPartLevel0) {  // the pyramid belongs to the upper-level part
+            lv[l] = c->level[l];
+            lp[l] = (size_t)c->levelW[l];
+            continue;
+        }
         LaunchScope ls(c, pq, F_PYR,
                        (double)c->levelW[l - 1] * c->levelH[l - 1] + (double)c->levelW[l] * c->levelH[l]);
         HIPCHK(c, bimpk::launch_pyr_down(lv[l - 1], c->levelW[l - 1], c->levelH[l - 1], lp[l - 1],
@@ -336,6 +358,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
     for (int pass = useSide ? 0 : 1; pass < 2; ++pass) {
         for (const Step& st : c->steps) {
             if (useSide && on_side(st) != (pass == 0)) continue;
+            if ((part == kPartUpper && !on_side(st)) || (part == kPartLevel0 && on_side(st)))
+                continue;
             const int r = run_step(st);
             if (r) return r;
         }
@@ -401,8 +425,18 @@ int ensure_kp_buf(bimp_ctx* c, size_t bytes) {
 }
 
 // Runs NMS into `out` (host or device) and reads the count back.
+// With `frame`, the whole frame (filtering and NMS, eagerly or as the frame's graph) is issued
+// into the keypoint target instead of the NMS alone.
+struct FrameIn {
+    const uint8_t* img;
+    size_t pitch;
+    int on_device;
+};
+int run_frame_any(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, void* out,
+                  int cap, int ld, hipStream_t s);
+
 int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_device, int* count,
-                  hipStream_t s, void* staging = nullptr) {
+                  hipStream_t s, void* staging = nullptr, const FrameIn* frame = nullptr) {
     const size_t elem = matrix_ld > 0 ? 0 : sizeof(bimp_keypoint);
     void* target = out;
     if (!out_on_device && cap > 0 && staging) {
@@ -414,7 +448,9 @@ int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_dev
         if (st) return st;
         target = c->kpBuf;
     }
-    int st = run_keypoints(c, target, cap, matrix_ld, s);
+    int st = frame ? run_frame_any(c, frame->img, frame->pitch, frame->on_device, target, cap,
+                                   matrix_ld, s)
+                   : run_keypoints(c, target, cap, matrix_ld, s);
     if (st) return st;
     int total = 0;
     HIPCHK(c, hipMemcpyAsync(&total, c->counter, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -435,6 +471,123 @@ int keypoints_out(bimp_ctx* c, void* out, int cap, int matrix_ld, int out_on_dev
                  std::to_string(cap) + ")";
         return BIMP_W_OVERFLOW;
     }
+    return BIMP_OK;
+}
+
+// ---- per-frame graph (BIMP_FLAG_GRAPH)
+// The frame is captured as separate single-stream graphs, composed into one graph whose
+// dependencies say what the second stream's events say in the eager form: the per-frame reset
+// first, then the work above level 0 (pyramid included) and the level-0 work as two
+// independent branches, then the keypoint pass.  Host frames are uploaded to the context's
+// level-0 buffer before the graph is launched; the graph reads that buffer.
+int capture_part(bimp_ctx* c, hipGraph_t* g, int (*body)(bimp_ctx*, hipStream_t, void*),
+                 void* arg) {
+    HIPCHK(c, hipStreamBeginCapture(c->capStream, hipStreamCaptureModeThreadLocal));
+    const int st = body(c, c->capStream, arg);
+    hipGraph_t out = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->capStream, &out);
+    if (st != BIMP_OK) {
+        if (out) (void)hipGraphDestroy(out);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    *g = out;
+    return BIMP_OK;
+}
+
+struct GraphArgs {
+    const uint8_t* src;
+    size_t pitch;
+    void* out;
+    int cap, ld, part;
+};
+
+int build_frame_graph(bimp_ctx* c, const uint8_t* src, size_t pitch, void* out, int cap, int ld) {
+    if (c->frameExec) {
+        (void)hipGraphExecDestroy(c->frameExec);
+        c->frameExec = nullptr;
+    }
+    if (!c->capStream) HIPCHK(c, hipStreamCreateWithFlags(&c->capStream, hipStreamNonBlocking));
+    GraphArgs ga{src, pitch, out, cap, ld, kPartHead};
+    auto filt = [](bimp_ctx* x, hipStream_t q, void* a) {
+        const GraphArgs* g = static_cast<const GraphArgs*>(a);
+        return run_filter(x, g->src, g->pitch, 1, true, q, g->part);
+    };
+    auto kp = [](bimp_ctx* x, hipStream_t q, void* a) {
+        const GraphArgs* g = static_cast<const GraphArgs*>(a);
+        return run_keypoints(x, g->out, g->cap, g->ld, q);
+    };
+    const bool split = c->side && !(c->opt.flags & BIMP_FLAG_SINGLE_STREAM);
+    hipGraph_t parts[4] = {nullptr, nullptr, nullptr, nullptr};  // head, upper, level 0, tail
+    auto release = [&]() {
+        for (auto& g : parts)
+            if (g) (void)hipGraphDestroy(g);
+    };
+    int st = capture_part(c, &parts[0], filt, &ga);
+    if (!st && split) {
+        ga.part = kPartUpper;
+        st = capture_part(c, &parts[1], filt, &ga);
+        ga.part = kPartLevel0;
+        if (!st) st = capture_part(c, &parts[2], filt, &ga);
+    } else if (!st) {
+        ga.part = kPartBody;
+        st = capture_part(c, &parts[2], filt, &ga);
+    }
+    if (!st) st = capture_part(c, &parts[3], kp, &ga);
+    if (st) {
+        release();
+        return st;
+    }
+    hipGraph_t g = nullptr;
+    hipGraphNode_t head = nullptr, upper = nullptr, l0 = nullptr, tail = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    if (e == hipSuccess) e = hipGraphAddChildGraphNode(&head, g, nullptr, 0, parts[0]);
+    if (e == hipSuccess && parts[1]) e = hipGraphAddChildGraphNode(&upper, g, &head, 1, parts[1]);
+    if (e == hipSuccess) e = hipGraphAddChildGraphNode(&l0, g, &head, 1, parts[2]);
+    if (e == hipSuccess) {
+        hipGraphNode_t deps[2] = {l0, upper};
+        e = hipGraphAddChildGraphNode(&tail, g, deps, upper ? 2 : 1, parts[3]);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&c->frameExec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    release();
+    if (e != hipSuccess) {
+        c->frameExec = nullptr;
+        return hip_fail(c, e, "frame graph");
+    }
+    c->gSrc = src;
+    c->gPitch = pitch;
+    c->gOut = out;
+    c->gCap = cap;
+    c->gLd = ld;
+    c->gFlags = c->opt.flags;
+    return BIMP_OK;
+}
+
+// One frame into `out` (device) on `s`: eager launches, or with BIMP_FLAG_GRAPH the frame's
+// graph (built on first use and whenever the frame or output pointers change).
+int run_frame_any(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, void* out,
+                  int cap, int ld, hipStream_t s) {
+    if (!(c->opt.flags & BIMP_FLAG_GRAPH) || (c->opt.flags & BIMP_FLAG_PROFILE)) {
+        int st = run_filter(c, img, pitch, on_device, true, s);
+        return st ? st : run_keypoints(c, out, cap, ld, s);
+    }
+    const uint8_t* src = img;
+    size_t p = pitch;
+    if (!on_device) {
+        // the graph reads the level-0 buffer; the upload is ordered before it on s
+        const int st = upload_frame(c, img, pitch, s);
+        if (st) return st;
+        src = c->level[0];
+        p = (size_t)c->width;
+    }
+    if (!c->frameExec || c->gSrc != src || c->gPitch != p || c->gOut != out || c->gCap != cap ||
+        c->gLd != ld || c->gFlags != c->opt.flags) {
+        const int st = build_frame_graph(c, src, p, out, cap, ld);
+        if (st) return st;
+    }
+    c->maxValid = true;
+    HIPCHK(c, hipGraphLaunch(c->frameExec, s));
     return BIMP_OK;
 }
 
@@ -850,9 +1003,8 @@ int bimp_detect_u8(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_dev
         return fail(c, BIMP_E_ARG, "bad arguments");
     hipStream_t s = (hipStream_t)stream;
     c->lastStream = s;
-    int st = run_filter(c, img, pitch, img_on_device, true, s);
-    if (st) return st;
-    return keypoints_out(c, out, cap, 0, out_on_device, count, s);
+    const FrameIn fr{img, pitch, img_on_device};
+    return keypoints_out(c, out, cap, 0, out_on_device, count, s, nullptr, &fr);
 }
 
 int bimp_detect_u8_noalloc(bimp_ctx* c, const uint8_t* img, size_t pitch, int img_on_device,
@@ -884,9 +1036,8 @@ int bimp_detect_u8_matrix(bimp_ctx* c, const uint8_t* img, size_t pitch, int img
         return fail(c, BIMP_E_ARG, "bad arguments");
     hipStream_t s = (hipStream_t)stream;
     c->lastStream = s;
-    int st = run_filter(c, img, pitch, img_on_device, true, s);
-    if (st) return st;
-    return keypoints_out(c, mat, cap, ld, mat_on_device, count, s);
+    const FrameIn fr{img, pitch, img_on_device};
+    return keypoints_out(c, mat, cap, ld, mat_on_device, count, s, nullptr, &fr);
 }
 
 int bimp_get_map(bimp_ctx* c, int i, int which, int o, float* dst, size_t pitch, int dst_on_device,
@@ -1106,8 +1257,7 @@ int bimp_time_detect(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bimp_keypo
     HIPCHK(c, hipEventCreate(&b));
     HIPCHK(c, hipEventRecord(a, s));
     for (int it = 0; it < iters; ++it) {
-        int st = run_filter(c, d_img, pitch, 1, true, s);
-        if (!st) st = run_keypoints(c, d_out, cap, 0, s);
+        int st = run_frame_any(c, d_img, pitch, 1, d_out, cap, 0, s);
         if (st) {
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
@@ -1141,8 +1291,7 @@ int bimp_time_detect_frames(bimp_ctx* c, const uint8_t* d_img, size_t pitch, bim
     }
     hipError_t r = hipEventRecord(ev[0], s);
     for (int it = 0; it < iters && r == hipSuccess; ++it) {
-        int st = run_filter(c, d_img, pitch, 1, true, s);
-        if (!st) st = run_keypoints(c, d_out, cap, 0, s);
+        int st = run_frame_any(c, d_img, pitch, 1, d_out, cap, 0, s);
         if (st) {
             (void)hipStreamSynchronize(s);
             release();

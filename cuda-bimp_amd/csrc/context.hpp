@@ -108,6 +108,14 @@ struct bimp_ctx {
     size_t tCap2 = 0;
     // this device's copy of the static keypoint staging of bimp_detect_u8_noalloc
     bimp_keypoint* noallocStaging = nullptr;
+    // BIMP_FLAG_GRAPH: the frame as one executable graph, rebuilt when its frame or output
+    // pointers change (captured on capStream, a stream of the context's own)
+    hipStream_t capStream = nullptr;
+    hipGraphExec_t frameExec = nullptr;
+    const void* gSrc = nullptr;
+    size_t gPitch = 0;
+    const void* gOut = nullptr;
+    int gCap = -1, gLd = -1, gFlags = -1;
 };
 
 namespace bimpi {

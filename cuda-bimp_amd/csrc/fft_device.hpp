@@ -301,79 +301,109 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
     }
 }
 
-// LDS floats one group's exchange region needs.
+// LDS floats one group's exchange region needs, and the address of element i in it.  After
+// the first stage lane j holds its R outputs j*R .. j*R+R-1.  When R <= VMAX (a parameter of
+// the transform) they leave as R/4 float4 writes: consecutive lanes' 16-byte writes 4R bytes
+// apart, one instruction for four scalar writes and no per-element address arithmetic.
+// Otherwise each is a scalar write whose address XOR-swizzles the bank bits of its 32-word row
+// with the row index (a bijection within the row: it spreads the stride-R writes over all
+// banks and keeps the reads of 32 consecutive elements conflict-free, at no extra LDS).  The
+// float4 form is faster for R = 16 in the column and forward row passes and slower in the
+// inverse row passes, which therefore keep the swizzle for R = 16.  Later exchanges write runs
+// of >= R consecutive lanes and keep the plain layout.
+template <int Ns, int R, int VMAX>
+inline constexpr bool kVecFirst = Ns == 1 && R % 4 == 0 && R <= VMAX;
+
 template <int N>
 __host__ __device__ constexpr int fft_lds_floats() { return N; }
-
-// The exchange moves real parts, then imaginary parts, through an LDS region of
-// fft_lds_floats<N>() floats: a register slot keeps its imaginary part until the second half
-// has been written, so half the LDS of a complex exchange suffices.
-//
-// After the first stage (Ns = 1) lane j holds the R consecutive outputs j*R .. j*R+R-1: they
-// leave as R/4 float4 writes (every first radix is 4, 8 or 16), so consecutive lanes' 16-byte
-// writes lie 4R bytes apart and one instruction replaces R scalar writes.  Later stages write
-// each element at (j/Ns)*Ns*R + j%Ns + r*Ns, whose r-dependent part is an immediate offset.
-// Reads are j + r*(N/R2) for the next radix R2: consecutive lanes, consecutive words.
-template <int N, int Ns, int R, int R2>
-__device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, int lane) {
-    using C = FFTCfg<N>;
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-        if constexpr (Ns == 1) {
-            static_assert(R % 4 == 0, "first radix 4, 8 or 16");
-#pragma unroll
-            for (int m = 0; m < C::E / R; ++m) {
-                float4* dst = reinterpret_cast<float4*>(lds + (lane + C::TPV * m) * R);
-#pragma unroll
-                for (int q = 0; q < R / 4; ++q) {
-                    const float2* p = &v[m * R + 4 * q];
-                    dst[q] = half ? make_float4(p[0].y, p[1].y, p[2].y, p[3].y)
-                                  : make_float4(p[0].x, p[1].x, p[2].x, p[3].x);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < C::E / R; ++m) {
-                const int j = lane + C::TPV * m;
-                float* dst = lds + (j / Ns) * Ns * R + (j % Ns);
-#pragma unroll
-                for (int r = 0; r < R; ++r) dst[r * Ns] = half ? v[m * R + r].y : v[m * R + r].x;
-            }
-        }
-        stage_sync();
-#pragma unroll
-        for (int m = 0; m < C::E / R2; ++m) {
-            const float* src = lds + lane + C::TPV * m;
-#pragma unroll
-            for (int r = 0; r < R2; ++r) {
-                const float x = src[r * (N / R2)];
-                if (half)
-                    v[m * R2 + r].y = x;
-                else
-                    v[m * R2 + r].x = x;
-            }
-        }
-        stage_sync();
-    }
+template <int Ns, int R, int VMAX>
+__device__ __forceinline__ int lds_pad(int i) {
+    if constexpr (Ns == 1 && !kVecFirst<Ns, R, VMAX>)
+        return (i & ~31) | ((i ^ (i >> 5)) & 31);
+    else
+        return i;
 }
 
-template <int N, bool INV, int Ns, class L>
+// The exchange moves real parts, then imaginary parts, through an LDS region of
+// fft_lds_floats<N>() floats: a register slot keeps its old imaginary part until the second
+// half has been written, so half the LDS of a complex exchange suffices.
+template <int N, int Ns, int R, int R2, int VMAX>
+__device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, int lane) {
+    using C = FFTCfg<N>;
+    if constexpr (kVecFirst<Ns, R, VMAX>) {
+#pragma unroll
+        for (int m = 0; m < C::E / R; ++m) {
+            float4* dst = reinterpret_cast<float4*>(lds + (lane + C::TPV * m) * R);
+#pragma unroll
+            for (int q = 0; q < R / 4; ++q)
+                dst[q] = make_float4(v[m * R + 4 * q].x, v[m * R + 4 * q + 1].x,
+                                     v[m * R + 4 * q + 2].x, v[m * R + 4 * q + 3].x);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < C::E / R; ++m) {
+            const int j = lane + C::TPV * m;
+            const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R, VMAX>(base + r * Ns)] = v[m * R + r].x;
+        }
+    }
+    stage_sync();
+    float im[C::E];
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) im[e] = v[e].y;
+#pragma unroll
+    for (int m = 0; m < C::E / R2; ++m) {
+        const int j = lane + C::TPV * m;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[lds_pad<Ns, R, VMAX>(j + r * (N / R2))];
+    }
+    stage_sync();
+    if constexpr (kVecFirst<Ns, R, VMAX>) {
+#pragma unroll
+        for (int m = 0; m < C::E / R; ++m) {
+            float4* dst = reinterpret_cast<float4*>(lds + (lane + C::TPV * m) * R);
+#pragma unroll
+            for (int q = 0; q < R / 4; ++q)
+                dst[q] = make_float4(im[m * R + 4 * q], im[m * R + 4 * q + 1],
+                                     im[m * R + 4 * q + 2], im[m * R + 4 * q + 3]);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < C::E / R; ++m) {
+            const int j = lane + C::TPV * m;
+            const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R, VMAX>(base + r * Ns)] = im[m * R + r];
+        }
+    }
+    stage_sync();
+#pragma unroll
+    for (int m = 0; m < C::E / R2; ++m) {
+        const int j = lane + C::TPV * m;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[lds_pad<Ns, R, VMAX>(j + r * (N / R2))];
+    }
+    stage_sync();
+}
+
+template <int N, bool INV, int Ns, class L, int VMAX>
 struct Stages;
-template <int N, bool INV, int Ns, int R, int... Rest>
-struct Stages<N, INV, Ns, RadList<R, Rest...>> {
+template <int N, bool INV, int Ns, int R, int... Rest, int VMAX>
+struct Stages<N, INV, Ns, RadList<R, Rest...>, VMAX> {
     // tw points at this stage's rows (unused for the first stage)
     template <class TW>
     static __device__ __forceinline__ void run(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
                                                TW tw) {
         butterflies<N, INV, Ns, R>(v, lane, tw);
         if constexpr (sizeof...(Rest) > 0) {
-            exchange<N, Ns, R, RadInfo<RadList<Rest...>>::first>(v, lds, lane);
+            exchange<N, Ns, R, RadInfo<RadList<Rest...>>::first, VMAX>(v, lds, lane);
             constexpr int next_ns = Ns * R;
             constexpr int adv = Ns > 1 ? Ns * tw_row(R) : 0;  // float2
             if constexpr (std::is_same<TW, LdsTw>::value)
-                Stages<N, INV, next_ns, RadList<Rest...>>::run(v, lds, lane, LdsTw{tw.p + adv / 2});
+                Stages<N, INV, next_ns, RadList<Rest...>, VMAX>::run(v, lds, lane, LdsTw{tw.p + adv / 2});
             else
-                Stages<N, INV, next_ns, RadList<Rest...>>::run(v, lds, lane, tw + adv);
+                Stages<N, INV, next_ns, RadList<Rest...>, VMAX>::run(v, lds, lane, tw + adv);
         }
     }
 };
@@ -415,16 +445,16 @@ __device__ __forceinline__ void tw_to_lds(const float2* tw, float4* dst, int tid
 // The twiddle pointer is passed through an empty asm statement so that a kernel running
 // several transforms in a loop reloads twiddles (L1 hits) instead of keeping every one of
 // them live in registers across iterations.
-template <int N, bool INV>
+template <int N, bool INV, int VMAX = 16>
 __device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
                                     const float2* tw) {
     asm volatile("" : "+s"(tw));
-    Stages<N, INV, 1, typename FFTCfg<N>::Rad>::run(v, lds, lane, tw);
+    Stages<N, INV, 1, typename FFTCfg<N>::Rad, VMAX>::run(v, lds, lane, tw);
 }
-template <int N, bool INV>
+template <int N, bool INV, int VMAX = 16>
 __device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane, LdsTw tw) {
     asm volatile("" : "+s"(tw.p));
-    Stages<N, INV, 1, typename FFTCfg<N>::Rad>::run(v, lds, lane, tw);
+    Stages<N, INV, 1, typename FFTCfg<N>::Rad, VMAX>::run(v, lds, lane, tw);
 }
 
 }  // namespace bimpdev

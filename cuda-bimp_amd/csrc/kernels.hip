@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __re
     float2 v[C::E];
     load_t_row<FW>(v, Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW, Hc, active ? y : 0,
                    tsh, lane);
-    fft<FW, true>(v, lds + (size_t)g * fft_lds_floats<FW>(), lane, tw);
+    fft<FW, true, 8>(v, lds + (size_t)g * fft_lds_floats<FW>(), lane, tw);
     if (active) {
         float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
 #pragma unroll
@@ -421,7 +421,7 @@ __global__ void __launch_bounds__(InvRowCfg<FW>::THREADS) k_inv_rows_p(
         float2 v[C::E];
         load_t_row<FW>(v, Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW, Hc,
                        active ? y : 0, tsh, lane);
-        fft<FW, true>(v, my, lane, LdsTw{(const __attribute__((address_space(3))) float4*)twl});
+        fft<FW, true, 8>(v, my, lane, LdsTw{(const __attribute__((address_space(3))) float4*)twl});
         if (active) {
             float* out = ss.C[sidx] + ((size_t)o * Hc + y) * ldc;
 #pragma unroll

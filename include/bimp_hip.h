@@ -88,6 +88,15 @@ typedef struct bimp_options {
  * is made, and the precision form keeps one direct launch for all levels) and at every frame.
  * The batching form's contexts set it. */
 #define BIMP_FLAG_SINGLE_STREAM 2
+/* Replay each frame of bimp_detect_u8, bimp_detect_u8_matrix and bimp_time_detect(_frames)
+ * as one hipGraph instead of issuing its ~30 launches (no counterpart in the reference).  The
+ * graph is built on the first such frame and again whenever the frame or output pointers, the
+ * capacity or the flags change; host frames are uploaded to the context's own level-0 buffer,
+ * so successive host frames reuse one graph.  With a second stream the graph has two
+ * independent branches (the work above pyramid level 0 and the level-0 work) between the
+ * per-frame reset and the keypoint pass.  Ignored while BIMP_FLAG_PROFILE is set; not used
+ * by bimp_detect_u8_noalloc. */
+#define BIMP_FLAG_GRAPH 4
 
 /* Arithmetic of the Gabor bank.  FP32: FFT convolution in FP32 (the reference's form).
  * BF16 / FP16: direct spatial convolution on the matrix cores, uint8 samples times taps
