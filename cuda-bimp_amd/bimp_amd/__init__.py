@@ -26,6 +26,7 @@ MAP_DOUBLE, MAP_SINGLE, MAP_COMPLEX = 0, 1, 2
 FLAG_PROFILE = 1
 FLAG_SINGLE_STREAM = 2
 FLAG_GRAPH = 4
+FLAG_RESIZE_CPU = 8
 PRECISION_FP32, PRECISION_BF16, PRECISION_FP16 = 0, 1, 2
 DIRECT_MAX_KERNEL = 55
 

@@ -230,7 +230,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                        (double)c->levelW[l - 1] * c->levelH[l - 1] + (double)c->levelW[l] * c->levelH[l]);
         HIPCHK(c, bimpk::launch_pyr_down(lv[l - 1], c->levelW[l - 1], c->levelH[l - 1], lp[l - 1],
                                          c->level[l], c->levelW[l], c->levelH[l],
-                                         (size_t)c->levelW[l], pq));
+                                         (size_t)c->levelW[l],
+                                         (c->opt.flags & BIMP_FLAG_RESIZE_CPU) != 0, pq));
         lv[l] = c->level[l];
         lp[l] = (size_t)c->levelW[l];
     }

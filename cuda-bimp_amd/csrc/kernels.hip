@@ -33,10 +33,13 @@ __device__ __forceinline__ int reflect101(int i, int n) {
     return i;
 }
 
-// 5x5 binomial [1 4 6 4 1]^2 / 256, reflect-101 border, round half to even: the float sum is
-// exact, so this equals OpenCV-CUDA pyrDown's saturate_cast of its FP32 sum.
+// 5x5 binomial [1 4 6 4 1]^2 / 256, reflect-101 border.  Round half to even: the float sum is
+// exact, so this equals OpenCV-CUDA pyrDown's saturate_cast of its FP32 sum
+// (cuda_bimp.cu:476).  half_up: (S + 128) >> 8, the fixed-point rounding of OpenCV's host
+// pyrDown, which the reference's resize_cpu option uses (cuda_bimp.cu:660-665).
 __global__ void k_pyr_down(const uint8_t* __restrict__ src, int sw, int sh, size_t spitch,
-                           uint8_t* __restrict__ dst, int dw, int dh, size_t dpitch) {
+                           uint8_t* __restrict__ dst, int dw, int dh, size_t dpitch,
+                           int half_up) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= dw || y >= dh) return;
@@ -50,7 +53,8 @@ __global__ void k_pyr_down(const uint8_t* __restrict__ src, int sw, int sh, size
         for (int j = 0; j < 5; ++j) racc += wt[j] * row[reflect101(2 * x + j - 2, sw)];
         acc += wt[i] * racc;
     }
-    dst[(size_t)y * dpitch + x] = (uint8_t)rintf((float)acc * (1.0f / 256.0f));
+    dst[(size_t)y * dpitch + x] =
+        half_up ? (uint8_t)((acc + 128) >> 8) : (uint8_t)rintf((float)acc * (1.0f / 256.0f));
 }
 
 // ------------------------------------------------------------------ forward rows
@@ -1175,9 +1179,9 @@ hipError_t launch_init(unsigned* maxslots, int nslots, int* counter, hipStream_t
 }
 
 hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, uint8_t* dst,
-                           int dw, int dh, size_t dpitch, hipStream_t s) {
+                           int dw, int dh, size_t dpitch, bool half_up, hipStream_t s) {
     dim3 grid((dw + 127) / 128, dh);
-    k_pyr_down<<<grid, 128, 0, s>>>(src, sw, sh, spitch, dst, dw, dh, dpitch);
+    k_pyr_down<<<grid, 128, 0, s>>>(src, sw, sh, spitch, dst, dw, dh, dpitch, half_up ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -100,8 +100,9 @@ hipError_t prepare_size(int n);
 hipError_t prepare_common();
 
 hipError_t launch_init(unsigned* maxslots, int nslots, int* counter, hipStream_t s);
+// half_up: round (S + 128) >> 8 as OpenCV's host pyrDown; else half to even as its CUDA one
 hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, uint8_t* dst,
-                           int dw, int dh, size_t dpitch, hipStream_t s);
+                           int dw, int dh, size_t dpitch, bool half_up, hipStream_t s);
 hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W,
                            const ScaleSet& ss, float scale, float2* R, int ldr, size_t rstride,
                            const float2* tw, hipStream_t s);

@@ -186,7 +186,8 @@ public:
             std::vector<float> lambdas = cuda::makeLambdasLog(4, 64, 2),
             const bimp_options* opt = nullptr, int channels = 1)
         : cuda_data(width, height, lambdas, opt), lambdas_(std::move(lambdas)),
-          width_(width), height_(height) {
+          width_(width), height_(height), base_flags_(initial_flags(opt)),
+          flags_(base_flags_) {
         loadImage(img, width, height, pitch, channels);
     }
 
@@ -353,10 +354,29 @@ public:
 
     cuda::CudaStuff cuda_data;
     bool has_changed_keypoints = true;
+    // Context::resize_cpu (bimp.hpp:24): in the CPU keypoint mode, pyramid levels rounded as
+    // the host cv::pyrDown rounds (BIMP_FLAG_RESIZE_CPU, computed on the device); the
+    // reference reads it in that mode only (cuda_bimp.cu:113-124).
+    bool resize_cpu = false;
 
 private:
+    static int initial_flags(const bimp_options* opt) {
+        if (opt) return opt->flags;
+        bimp_options d;
+        bimp_default_options(&d);
+        return d.flags;
+    }
     void refresh() {
+        if (resize_cpu != last_resize_cpu_) {
+            last_resize_cpu_ = resize_cpu;
+            if (cpu_mode_) has_changed_keypoints = true;
+        }
         if (!has_changed_keypoints) return;
+        const int want = base_flags_ | (cpu_mode_ && resize_cpu ? BIMP_FLAG_RESIZE_CPU : 0);
+        if (want != flags_) {
+            check(bimp_set_flags(cuda_data.ctx, want), cuda_data.ctx);
+            flags_ = want;
+        }
         const int cap = BIMP_KP_MAX_POINTS;
         if (cpu_mode_) {
             check(bimp_filter_u8(cuda_data.ctx, d_image_, image_pitch_, 1, nullptr), cuda_data.ctx);
@@ -406,6 +426,8 @@ private:
     bool cpu_mode_ = false;
     std::vector<float> lambdas_;
     int width_ = 0, height_ = 0;
+    int base_flags_ = 0, flags_ = 0;  // the options' flags; those the context has now
+    bool last_resize_cpu_ = false;
     DeviceBytes raw_, image_, color_, matrix_d_, desc_, surf_, integral_;
     DeviceDescriptors device_desc_, device_surf_;
     bool has_changed_descriptors_ = true, has_changed_surf_ = true;

@@ -97,6 +97,12 @@ typedef struct bimp_options {
  * per-frame reset and the keypoint pass.  Ignored while BIMP_FLAG_PROFILE is set; not used
  * by bimp_detect_u8_noalloc. */
 #define BIMP_FLAG_GRAPH 4
+/* Pyramid levels rounded as OpenCV's host cv::pyrDown rounds, (S + 128) >> 8 (half up), in
+ * place of cv::cuda::pyrDown's round half to even: the reference's resize_cpu option
+ * (setupCUDAImages, cuda_bimp.cu:649-687, selected by getKeypoints(..., cpu_mode, resize_cpu),
+ * cuda_bimp.hpp:109-111, and Context::resize_cpu, bimp.hpp:24).  Computed on the GPU.  The
+ * reference honours it only in its CPU keypoint mode; the C++ facade sets it only there. */
+#define BIMP_FLAG_RESIZE_CPU 8
 
 /* Arithmetic of the Gabor bank.  FP32: FFT convolution in FP32 (the reference's form).
  * BF16 / FP16: direct spatial convolution on the matrix cores, uint8 samples times taps

@@ -51,6 +51,8 @@ int bo_make_lambdas_log(float start, float end, int spo, float* out, int cap);
 int bo_scale_params(float lambda2, int width, int height, bo_scale* s);
 void bo_gabor(const bo_scale* s, int o, float* ke, float* ko);
 int bo_pyr_down(const uint8_t* src, int w, int h, size_t pitch, uint8_t* dst);
+/* OpenCV's host cv::pyrDown rounding, (S + 128) >> 8: the reference's resize_cpu option */
+int bo_pyr_down_host(const uint8_t* src, int w, int h, size_t pitch, uint8_t* dst);
 void bo_pad_clamp(const uint8_t* img, size_t pitch, const bo_scale* s, float* out);
 int bo_end_stop_offsets(float lambda, int o, int* a, int* b);
 int bo_inhibition_offsets(float lambda, int o, int* a, int* b);
@@ -68,6 +70,8 @@ void bo_brute_filter(const uint8_t* level_img, size_t pitch, const bo_scale* s, 
 /* ---- context: bank precomputed at creation, as CudaStuff does ---- */
 typedef struct bo_ctx bo_ctx;
 bo_ctx* bo_create(int width, int height, const float* lambdas, int n, int fp64);
+/* The context's pyramid (bo_set_image) rounded as bo_pyr_down_host when on != 0 */
+int bo_set_resize_cpu(bo_ctx* c, int on);
 void bo_destroy(bo_ctx* c);
 int bo_num_scales(const bo_ctx* c);
 int bo_get_scale(const bo_ctx* c, int i, bo_scale* out);

@@ -6,8 +6,8 @@
 //   writes: int32 prepared frame width and height; int32 count, count keypoint records, the
 //   SURF_CUDA matrix (7 x count, from the device); the response views' scale count and, per
 //   scale, the D map read through its device view; int32 count and records of the CPU
-//   keypoint mode; count x 152 descriptor floats (SURF, zeros, colour histogram); the
-//   matches of the context against itself.
+//   keypoint mode, then of the CPU mode with resize_cpu; count x 152 descriptor floats (SURF,
+//   zeros, colour histogram); the matches of the context against itself.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -64,6 +64,13 @@ int main(int argc, char** argv) {
         const int nc = (int)cpu.size();
         put_int(nc);
         out.write(reinterpret_cast<const char*>(cpu.data()), (std::streamsize)(nc * sizeof(bimp::KeyPoint)));
+        // Context::resize_cpu in the CPU mode: host-rounded pyramid levels
+        ctx.resize_cpu = true;
+        const std::vector<bimp::KeyPoint> cpu_r = ctx.getKeypointsAndDownload();
+        put_int((int)cpu_r.size());
+        out.write(reinterpret_cast<const char*>(cpu_r.data()),
+                  (std::streamsize)(cpu_r.size() * sizeof(bimp::KeyPoint)));
+        ctx.resize_cpu = false;
         ctx.setCPUMode(false);
         // getDescriptors: count x BIMP_DIMENSIONS (SURF, zeros, colour histogram)
         const std::vector<float> desc = ctx.getDescriptorsAndDownload();

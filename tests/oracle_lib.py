@@ -45,6 +45,8 @@ def lib():
         L.bo_gabor.argtypes = [C.POINTER(Scale), C.c_int, P, P]
         L.bo_gabor.restype = None
         L.bo_pyr_down.argtypes = [P, C.c_int, C.c_int, C.c_size_t, P]
+        L.bo_pyr_down_host.argtypes = [P, C.c_int, C.c_int, C.c_size_t, P]
+        L.bo_set_resize_cpu.argtypes = [P, C.c_int]
         L.bo_pad_clamp.argtypes = [P, C.c_size_t, C.POINTER(Scale), P]
         L.bo_pad_clamp.restype = None
         L.bo_end_stop_offsets.argtypes = [C.c_float, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -177,11 +179,14 @@ def direct_taps(s, nsig=0.0, fp16=False):
     return ext.value, fscale.value, dc, taps
 
 
-def pyr_down(img):
+def pyr_down(img, half_up=False):
+    """One pyramid level: cv::cuda::pyrDown rounding (half to even), or with half_up the host
+    cv::pyrDown rounding of the reference's resize_cpu option."""
     h, w = img.shape
     img = np.ascontiguousarray(img)
     out = np.zeros(((h + 1) // 2, (w + 1) // 2), np.uint8)
-    assert lib().bo_pyr_down(ptr(img), w, h, w, ptr(out)) == 0
+    fn = lib().bo_pyr_down_host if half_up else lib().bo_pyr_down
+    assert fn(ptr(img), w, h, w, ptr(out)) == 0
     return out
 
 
@@ -237,6 +242,10 @@ class Oracle:
         if getattr(self, "c", None):
             lib().bo_destroy(self.c)
             self.c = None
+
+    def set_resize_cpu(self, on=True):
+        """Pyramid levels rounded as the reference's resize_cpu option (host pyrDown)."""
+        assert lib().bo_set_resize_cpu(self.c, 1 if on else 0) == 0
 
     def set_image(self, img):
         img = np.ascontiguousarray(img, np.uint8)

@@ -87,6 +87,13 @@ def test_context_facade_matches_c_abi(tmp_path, exe, w, h, cn, bank):
     want_cpu, mc = cs.keypoints_cpu_mode()
     assert nc == mc and nc > 0
     assert np.array_equal(bits(cpu), bits(want_cpu))
+    nr = r.int()
+    cpu_r = r.take(nr, B.KEYPOINT_DTYPE)
+    cs.set_flags(B.FLAG_RESIZE_CPU)  # the host-rounded pyramid of Context::resize_cpu
+    cs.detect(np.ascontiguousarray(prepared[:h, :w]))
+    want_r, mr = cs.keypoints_cpu_mode()
+    cs.set_flags(0)
+    assert nr == mr and np.array_equal(bits(cpu_r), bits(want_r))
     desc = r.take(n * 152, np.float32).reshape(n, 152)
     surf, _ = B.surf_descriptors(prepared, B.keypoints_to_matrix(want))
     assert np.array_equal(desc[:, :64].view(np.uint32), surf.view(np.uint32))

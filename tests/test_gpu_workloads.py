@@ -10,6 +10,9 @@
   crop of the upper scales (cuda_bimp.cu:473-486).
 - Every pyramid level bit-exact against the oracle's pyrDown (5-tap binomial, reflect-101,
   round half to even), on even and odd sizes.
+- The resize_cpu pyramid (BIMP_FLAG_RESIZE_CPU: host pyrDown rounding, half up): every level
+  bit-exact against the oracle's host-rounded chain, and detection end to end against the
+  oracle built on that pyramid.
 - The differentiation form at its workload: 16 FP32 512x512 images, 8 scales.
 """
 import os
@@ -86,9 +89,12 @@ def test_precision_form_1080p_against_fp64(B, frame1080, fmt, max_lambda):
 
 
 # ------------------------------------------------------------- default bank and odd sizes
-def _end_to_end(B, w, h, lam, nshapes, seed):
+def _end_to_end(B, w, h, lam, nshapes, seed, resize_cpu=False):
     cs = B.CudaStuff(w, h, lam)
     orc = O.Oracle(w, h, lam)
+    if resize_cpu:
+        cs.set_flags(B.FLAG_RESIZE_CPU)
+        orc.set_resize_cpu(True)
     img = B.synth_frame(w, h, nshapes, seed)
     kps, m, Ds, Ss = orc.detect(img, cap=1 << 16, want_maps=True)
     got, n = cs.detect(img, cap=1 << 16)
@@ -151,6 +157,27 @@ def test_pyramid_levels_bit_exact(B, w, h):
         assert bad == 0, level
     _log(f"{w}x{h}: {ties} half-way sums across the levels")
     assert ties > 0
+
+
+@pytest.mark.parametrize("w,h,nshapes,seed", [(641, 479, 40, 8), (1919, 1079, 300, 9)])
+def test_resize_cpu_pyramid(B, w, h, nshapes, seed):
+    """BIMP_FLAG_RESIZE_CPU (the reference's resize_cpu, cuda_bimp.cu:649-687): levels equal the
+    host-rounded oracle chain and differ from the device rounding where a half-way sum has an
+    even floor; the default bank's detection matches the oracle on that pyramid."""
+    lam = B.make_lambdas_log(4, 64, 2)
+    cs = _end_to_end(B, w, h, lam, nshapes, seed, resize_cpu=True)
+    img = B.synth_frame(w, h, nshapes, seed)
+    want, moved = img, 0
+    for level in range(1, max(si.level for si in cs.ks) + 1):
+        even = O.pyr_down(want)
+        want = O.pyr_down(want, half_up=True)
+        got = cs.get_level(level)
+        bad = np.count_nonzero(got != want)
+        moved += int(np.count_nonzero(want != even))
+        _log(f"{w}x{h} resize_cpu level {level}: {bad} differing pixels")
+        assert bad == 0, level
+    _log(f"{w}x{h}: {moved} level pixels where the two roundings differ")
+    assert moved > 0
 
 
 # ------------------------------------------------------- differentiation form, 16 x 512^2

@@ -155,6 +155,67 @@ def test_pyr_down_known_answers():
     assert np.all(O.pyr_down(const) == 77)
 
 
+def test_pyr_down_host_rounding_known_answers():
+    """The resize_cpu pyramid (OpenCV host pyrDown, (S + 128) >> 8): half-way sums round up."""
+    img = np.zeros((9, 9), np.uint8)
+    img[4, 4] = 128
+    d = O.pyr_down(img, half_up=True)
+    assert d[2, 2] == 18 and d[2, 1] == 3
+    assert d[1, 1] == 1    # 128 / 256 = 0.5 -> 1 (half to even gives 0)
+    img = np.zeros((9, 9), np.uint8)
+    img[4, 4] = 64
+    img[4, 5] = 64
+    assert O.pyr_down(img, half_up=True)[1, 2] == 3   # 2.5 -> 3 (half to even gives 2)
+    img[4, 5] = 0
+    assert O.pyr_down(img, half_up=True)[1, 2] == 2   # 1.5 -> 2 either way
+    img = np.zeros((9, 9), np.uint8)
+    img[2, 4] = 64
+    assert O.pyr_down(img, half_up=True)[0, 2] == 3   # reflect-101, as the device pyramid
+    const = np.full((7, 10), 255, np.uint8)
+    assert np.all(O.pyr_down(const, half_up=True) == 255)   # (65280 + 128) >> 8, no overflow
+
+
+def _pyr_numpy(img, half_up):
+    """Separable [1 4 6 4 1] passes on a reflect-101 padded copy (numpy's 'reflect' mode),
+    rounded from the integer sum: an independent statement of both pyramid roundings."""
+    k = np.array([1, 4, 6, 4, 1], np.int64)
+    p = np.pad(img.astype(np.int64), 2, mode="reflect")
+    h, w = img.shape
+    rows = sum(k[i] * p[i:i + h, :] for i in range(5))
+    S = sum(k[j] * rows[:, j:j + w] for j in range(5))[::2, ::2]
+    if half_up:
+        return ((S + 128) // 256).astype(np.uint8)
+    q, r = np.divmod(S, 256)
+    return (q + ((r > 128) | ((r == 128) & (q % 2 == 1)))).astype(np.uint8)
+
+
+@pytest.mark.parametrize("w,h,seed", [(2, 2, 0), (3, 5, 1), (17, 9, 2), (64, 48, 3),
+                                      (101, 77, 4)])
+def test_pyr_down_both_roundings_match_numpy(w, h, seed):
+    img = np.random.default_rng(seed).integers(0, 256, (h, w), dtype=np.uint8)
+    for half_up in (False, True):
+        assert np.array_equal(O.pyr_down(img, half_up), _pyr_numpy(img, half_up)), half_up
+    # the two differ exactly at half-way sums whose floor is even
+    a, b = O.pyr_down(img), O.pyr_down(img, half_up=True)
+    d = b.astype(int) - a.astype(int)
+    assert set(np.unique(d)) <= {0, 1}
+
+
+def test_oracle_context_resize_cpu_pyramid():
+    """bo_set_resize_cpu: the context's levels are the host-rounded chain."""
+    w, h = 161, 121
+    lam = O.BASE_BANK
+    img = O.synth_frame(w, h, 12, 5)
+    orc = O.Oracle(w, h, lam)
+    orc.set_resize_cpu(True)
+    orc.set_image(img)
+    for i, s in enumerate(orc.scales):
+        want = img
+        for _ in range(s.level):
+            want = O.pyr_down(want, half_up=True)
+        assert np.array_equal(orc.level_image(i), want[:s.dataH, :s.dataW]), i
+
+
 def test_pad_clamp_known_answers():
     s = O.scale_params(np.float32(4.0), 20, 10)  # kX = 14, FFT 16 x 32
     img = np.arange(200, dtype=np.uint8).reshape(10, 20)
