@@ -128,12 +128,24 @@ void free_all(bimp_ctx* c) {
     }
 }
 
-int launch_cells_scale(bimp_ctx* c, size_t i, hipStream_t s) {
-    auto& sc = c->sc[i];
-    const auto& g = sc.g;
-    LaunchScope ls(c, s, F_CELLS, 4.0 * kNumOri * (g.H + 1.0) * (g.W + 1.0) + 8.0 * g.H * g.W);
-    HIPCHK(c, bimpk::launch_cells(sc.C, sc.Hc, sc.ldc, g.FH, g.FW, g.H, g.W, sc.cp, sc.D, sc.S,
-                                  c->maxslots + i, s));
+// End-stop/inhibition of scales first .. first + n - 1, which are ready together: one launch
+// per (up to) bimpk::kCellMaxScales of them.
+int launch_cells_group(bimp_ctx* c, size_t first, int n, hipStream_t s) {
+    for (int k0 = 0; k0 < n; k0 += bimpk::kCellMaxScales) {
+        const int m = std::min(n - k0, bimpk::kCellMaxScales);
+        bimpk::CellScale cs[bimpk::kCellMaxScales];
+        double bytes = 0;
+        for (int k = 0; k < m; ++k) {
+            const size_t i = first + (size_t)(k0 + k);
+            auto& sc = c->sc[i];
+            const auto& g = sc.g;
+            cs[k] = bimpk::CellScale{sc.C, sc.Hc, sc.ldc, g.FH, g.FW, g.H, g.W, sc.cp, sc.D, sc.S,
+                                     c->maxslots + i};
+            bytes += 4.0 * kNumOri * (g.H + 1.0) * (g.W + 1.0) + 8.0 * g.H * g.W;
+        }
+        LaunchScope ls(c, s, F_CELLS, bytes);
+        HIPCHK(c, bimpk::launch_cells(cs, m, s));
+    }
     return BIMP_OK;
 }
 
@@ -266,10 +278,10 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                     HIPCHK(c, hipEventRecord(c->evCells, q));
                     HIPCHK(c, hipStreamWaitEvent(c->side, c->evCells, 0));
                 }
-                for (int k = 0; k < st.n; ++k) {
-                    const int r = launch_cells_scale(c, (size_t)(st.index + k), k < split ? q : c->side);
-                    if (r) return r;
-                }
+                int r = launch_cells_group(c, (size_t)st.index, split, q);
+                if (!r && split < st.n)
+                    r = launch_cells_group(c, (size_t)(st.index + split), st.n - split, c->side);
+                if (r) return r;
             }
             return BIMP_OK;
         }
@@ -343,10 +355,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                     HIPCHK(c, hipStreamWaitEvent(c->side, c->evCells, 0));
                     cq = c->side;
                 }
-                for (int k = 0; k < sub.n; ++k) {
-                    const int st = launch_cells_scale(c, (size_t)(gr.first + k0 + k), cq);
-                    if (st) return st;
-                }
+                const int st = launch_cells_group(c, (size_t)(gr.first + k0), sub.n, cq);
+                if (st) return st;
             }
         }
         return BIMP_OK;
@@ -374,10 +384,8 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
 }
 
 int run_cells(bimp_ctx* c, hipStream_t s) {
-    for (size_t i = 0; i < c->sc.size(); ++i) {
-        const int st = launch_cells_scale(c, i, s);
-        if (st) return st;
-    }
+    const int st = launch_cells_group(c, 0, (int)c->sc.size(), s);
+    if (st) return st;
     c->maxValid = true;
     return BIMP_OK;
 }

@@ -37,6 +37,9 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Global-memory pointers: loads and stores through them are global_* instructions, counted
+// by vmcnt only (through a generic pointer they are flat_* and also hold up LDS waits).
+#define BIMP_GLOBAL __attribute__((address_space(1)))
 
 template <class T>
 struct Mfma;
@@ -86,11 +89,11 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
     int js = njobs - 1;
     while (js > 0 && (int)blockIdx.x < jobs[js].tile0) --js;  // uniform
     const DirectJob& job = jobs[js];
-    const uint8_t* __restrict__ img = lv.img[job.level];
+    const BIMP_GLOBAL uint8_t* __restrict__ img = (const BIMP_GLOBAL uint8_t*)lv.img[job.level];
     const size_t pitch = lv.pitch[job.level];
     const DirectScale& ds = job.ds;
-    const v8* __restrict__ bfrag = reinterpret_cast<const v8*>(job.frag);
-    float* __restrict__ Cm = job.C;
+    const BIMP_GLOBAL v8* __restrict__ bfrag = (const BIMP_GLOBAL v8*)job.frag;
+    BIMP_GLOBAL float* __restrict__ Cm = (BIMP_GLOBAL float*)job.C;
     const int tb = (int)blockIdx.x - job.tile0;
     const int tby = tb / job.tiles_x, tbx = tb - tby * job.tiles_x;
     extern __shared__ unsigned char smem[];
@@ -126,7 +129,7 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
                 const bool rowin = r < TY + 2 * k;
                 const int gy = !rowin ? 0
                                       : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
-                const uint8_t* row = img + (size_t)gy * pitch;
+                const BIMP_GLOBAL uint8_t* row = img + (size_t)gy * pitch;
 #pragma unroll
                 for (int i = 0; i < kCols; ++i)
                     v[q][i] = (rowin && gxs[i] >= 0) ? (float)row[gxs[i]] : 0.f;
@@ -219,7 +222,7 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
         int vb, aoff, boff;
     };
     // filter fragments of u0 = 4*rho - j sit at row index u0 + M - 1
-    const v8* __restrict__ bbase = bfrag + (size_t)(M - 1) * nvb * 64 + lane;
+    const BIMP_GLOBAL v8* __restrict__ bbase = bfrag + (size_t)(M - 1) * nvb * 64 + lane;
     auto advance = [&](Cursor& c) {
         if (++c.vb == nvb) {
             c.vb = 0;
@@ -231,7 +234,7 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
         }
     };
     auto load_b = [&](const Cursor& c, v8* b) {
-        const v8* bro = bbase + c.boff;
+        const BIMP_GLOBAL v8* bro = bbase + c.boff;
 #pragma unroll
         for (int j = 0; j < M; ++j) b[j] = bro[-j * nvb * 64];
     };
@@ -254,37 +257,35 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
         for (int j = 0; j < M; ++j)
 #pragma unroll
             for (int g = 0; g < NCG; ++g) acc[j][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // Software-pipelined over the K-steps it = rho * nvb + vb: the image fragments of step
-        // it + 1 (LDS) and the filter fragments of step it + 2 (L2) are in flight while the
-        // MFMAs of step it run.  Fragment offsets advance incrementally (no division per step).
-        v8 a[NCG], b[M], b1[M];
-        Cursor ca{0, 0, 0};
-        load_a(ca, a);
-        load_b(ca, b);
-        Cursor cb = ca;
-        if (nsteps > 1) advance(cb);
-        load_b(cb, b1);
-        if (nsteps > 1) advance(ca);  // ca: step it + 1, cb: step it + 2 (both clamped)
-        if (nsteps > 2) advance(cb);
-#pragma unroll 1
-        for (int it = 0; it < nsteps; ++it) {
-            v8 an[NCG], bn[M];
-            load_a(ca, an);
-            load_b(cb, bn);
+        // Software-pipelined over the K-steps it = rho * nvb + vb with two fragment sets used
+        // in turn (the loop body is two steps): while the MFMAs of one step run, the image
+        // (LDS) and filter (L2) fragments of the next are loaded into the other set.  No
+        // fragment register is copied, so no wait falls on loads still in flight for a later
+        // step.  Fragment offsets advance incrementally (no division per step); the load after
+        // the last step repeats the last step's (clamped cursor, unused).
+        auto mfma_step = [&](const v8* a, const v8* b) {
 #pragma unroll
             for (int j = 0; j < M; ++j)
 #pragma unroll
                 for (int g = 0; g < NCG; ++g) acc[j][g] = Mfma<T>::run(a[g], b[j], acc[j][g]);
-#pragma unroll
-            for (int g = 0; g < NCG; ++g) a[g] = an[g];
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                b[j] = b1[j];
-                b1[j] = bn[j];
-            }
-            if (it + 2 < nsteps) advance(ca);
-            if (it + 3 < nsteps) advance(cb);
+        };
+        v8 a0[NCG], b0[M], a1[NCG], b1[M];
+        Cursor cl{0, 0, 0};
+        load_a(cl, a0);
+        load_b(cl, b0);
+        int it = 0;
+#pragma unroll 1
+        for (; it + 1 < nsteps; it += 2) {
+            advance(cl);  // step it + 1
+            load_a(cl, a1);
+            load_b(cl, b1);
+            mfma_step(a0, b0);
+            if (it + 2 < nsteps) advance(cl);  // step it + 2
+            load_a(cl, a0);
+            load_b(cl, b0);
+            mfma_step(a1, b1);
         }
+        if (it < nsteps) mfma_step(a0, b0);
 
         // Epilogue: lane holds filter jf = lane%16 for pixels 4*kb .. 4*kb+3 of each group;
         // the odd filter of orientation jf/2 sits in the neighbouring lane.  Over a pair of
@@ -307,15 +308,19 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
                 for (int q = 0; q < 4; ++q) {
                     const float mine = oddf ? acc[j][g2 + 1][q] : acc[j][g2][q];
                     const float send = oddf ? acc[j][g2][q] : acc[j][g2 + 1][q];
-                    const float other = __shfl_xor(send, 1, 64);
+                    // the neighbouring lane's value: DPP quad_perm [1, 0, 3, 2] (a VALU move,
+                    // no LDS crossbar round trip)
+                    const float other = __int_as_float(__builtin_amdgcn_update_dpp(
+                        0, __float_as_int(send), 0xB1, 0xF, 0xF, false));
                     const float e = ds.fscale * (oddf ? other : mine) - dce * bxs[q];
                     const float od = ds.fscale * (oddf ? mine : other) - dco * bxs[q];
                     c4[q] = ds.iscale * __builtin_amdgcn_sqrtf(e * e + od * od);
                 }
                 if (y < ds.Hc) {
-                    float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x;
+                    BIMP_GLOBAL float* out = Cm + ((size_t)o * ds.Hc + y) * ds.ldc + x;
                     if (x + 3 < ds.Wc) {
-                        *reinterpret_cast<float4*>(out) = make_float4(c4[0], c4[1], c4[2], c4[3]);
+                        *reinterpret_cast<BIMP_GLOBAL f32x4*>(out) =
+                            f32x4{c4[0], c4[1], c4[2], c4[3]};
                     } else {
 #pragma unroll
                         for (int q = 0; q < 4; ++q)

@@ -735,30 +735,55 @@ struct CellTiles {
     }
 };
 
-// One launch per scale: the first n_border workgroups take the border tiles (checked path),
-// the rest the interior rectangle (unchecked path, XCD-banded).
+// One scale of a cells launch: workgroups [block0, block0 + grid) of the launch, the first
+// n_border of them on the border tiles (checked path), the rest on the interior rectangle
+// (unchecked path, XCD-banded).
+struct CellJob {
+    MapView mv;
+    int H, W;
+    CellTileOffsets q;
+    CellTiles tg;
+    float* Dm;
+    float* Sm;
+    unsigned* maxslot;
+    int block0, grid;
+};
+struct CellJobs {
+    CellJob j[kCellMaxScales];
+    int n;
+};
+
+// Scales that are ready together share one launch (a job per scale, block0 ascending), so
+// that the small ones fill the last round of the large ones instead of leaving CUs idle.
 template <int HS>
-__global__ void __launch_bounds__(512) k_cells(MapView mv, int H, int W, CellTileOffsets q,
-                                               CellTiles tg, float* __restrict__ Dm,
-                                               float* __restrict__ Sm,
-                                               unsigned* __restrict__ maxslot) {
+__global__ void __launch_bounds__(512) k_cells(const CellJobs jobs) {
     extern __shared__ float4 tile4[];
     float* tile = reinterpret_cast<float*>(tile4);
     const int tid = threadIdx.y * kCellTW + threadIdx.x;
+    int js = jobs.n - 1;
+    while (js > 0 && (int)blockIdx.x < jobs.j[js].block0) --js;  // uniform
+    const CellJob& jb = jobs.j[js];
+    const int b = (int)blockIdx.x - jb.block0;
+    if (b >= jb.grid) return;  // alignment padding between jobs: uniform, before any barrier
+    const MapView& mv = jb.mv;
+    const CellTiles& tg = jb.tg;
+    const int H = jb.H, W = jb.W;
     float Dmax = -INFINITY;
     bool any = false;
     const int nb = tg.n_border();
-    if ((int)blockIdx.x < nb) {
+    if (b < nb) {
         int tx, ty;
-        tg.border_tile(blockIdx.x, tx, ty);
-        cells_tile<HS, false>(mv, H, W, q, tx * kCellTW, ty * kCellTH, tile, Dm, Sm, Dmax, any);
+        tg.border_tile(b, tx, ty);
+        cells_tile<HS, false>(mv, H, W, jb.q, tx * kCellTW, ty * kCellTH, tile, jb.Dm, jb.Sm,
+                              Dmax, any);
     } else {
-        const int t = xcd_band_index(blockIdx.x - nb, gridDim.x - nb);
+        const int t = xcd_band_index(b - nb, jb.grid - nb);
         const int nx = tg.ix1 - tg.ix0;
         if (t >= 0 && t < tg.n_interior())
-            cells_tile<HS, true>(mv, H, W, q, (tg.ix0 + t % nx) * kCellTW,
-                                 (tg.iy0 + t / nx) * kCellTH, tile, Dm, Sm, Dmax, any);
+            cells_tile<HS, true>(mv, H, W, jb.q, (tg.ix0 + t % nx) * kCellTW,
+                                 (tg.iy0 + t / nx) * kCellTH, tile, jb.Dm, jb.Sm, Dmax, any);
     }
+    unsigned* __restrict__ maxslot = jb.maxslot;
     // block maximum of D -> one atomic per workgroup
     unsigned key = any ? float_key(Dmax) : 0u;
 #pragma unroll
@@ -1267,11 +1292,9 @@ hipError_t launch_inv_rows(int FW, int FH, const ScaleSet& ss, const float2* T, 
 }
 
 template <int HS>
-static hipError_t launch_cells_hs(const MapView& mv, int FH, int FW, int H, int W,
-                                  const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
-                                  hipStream_t s) {
+static void make_cell_job(const MapView& mv, int FH, int FW, int H, int W, const CellParams& p,
+                          float* Dm, float* Sm, unsigned* maxslot, CellJob& job) {
     using CG = CellGeom<HS>;
-    const size_t lds = (size_t)2 * CG::TS * sizeof(float);
     CellTileOffsets q;
     for (int o = 0; o < kNumOri; ++o) {
         const int a = p.ea[o], b = p.eb[o];
@@ -1282,7 +1305,6 @@ static hipError_t launch_cells_hs(const MapView& mv, int FH, int FW, int H, int 
         q.exy[o] = 2 * abs(b) | (2 * abs(a)) << 16;
         q.ixy[o] = 2 * abs(p.ib[o]) | (2 * abs(p.ia[o])) << 16;
     }
-    dim3 block(kCellTW, kCellTY);
     CellTiles tg;
     tg.tiles_x = (W + kCellTW - 1) / kCellTW;
     tg.tiles_y = (H + kCellTH - 1) / kCellTH;
@@ -1299,22 +1321,54 @@ static hipError_t launch_cells_hs(const MapView& mv, int FH, int FW, int H, int 
         tg.ix0 = tg.ix1 = tg.iy0 = tg.iy1 = 0;  // everything takes the checked path
     const int nin = (tg.ix1 - tg.ix0) * (tg.iy1 - tg.iy0);
     const int nbd = tg.tiles_x * tg.tiles_y - nin;
-    const int grid = nbd + (nin + 7) / 8 * 8;
-    k_cells<HS><<<grid, block, lds, s>>>(mv, H, W, q, tg, Dm, Sm, maxslot);
+    job.mv = mv;
+    job.H = H;
+    job.W = W;
+    job.q = q;
+    job.tg = tg;
+    job.Dm = Dm;
+    job.Sm = Sm;
+    job.maxslot = maxslot;
+    job.grid = nbd + (nin + 7) / 8 * 8;
+}
+
+template <int HS>
+static hipError_t launch_cells_hs(const CellScale* sc, int n, hipStream_t s) {
+    using CG = CellGeom<HS>;
+    const size_t lds = (size_t)2 * CG::TS * sizeof(float);
+    CellJobs jobs;
+    jobs.n = n;
+    int blocks = 0;
+    for (int k = 0; k < n; ++k) {
+        const CellScale& c = sc[k];
+        make_cell_job<HS>(MapView{c.C, c.Hc, c.ldc, c.FH, c.FW}, c.FH, c.FW, c.H, c.W, c.p,
+                          c.Dm, c.Sm, c.maxslot, jobs.j[k]);
+        jobs.j[k].block0 = blocks;
+        blocks += (jobs.j[k].grid + 7) / 8 * 8;  // each job starts on a multiple of 8
+    }
+    k_cells<HS><<<blocks, dim3(kCellTW, kCellTY), lds, s>>>(jobs);
     return hipGetLastError();
+}
+
+hipError_t launch_cells(const CellScale* sc, int n, hipStream_t s) {
+    if (n < 1 || n > kCellMaxScales) return hipErrorInvalidValue;
+    // one halo for the launch: the largest the scales need (a larger one stages more, with
+    // the same results)
+    int h = 0;
+    for (int k = 0; k < n; ++k) h = std::max(h, cells_halo(sc[k].p));
+    const int hs = (h + 3) / 4 * 4;
+#define X(HS) \
+    if (hs <= HS) return launch_cells_hs<(HS > 0 ? HS : 4)>(sc, n, s);
+    BIMP_CELL_HALOS(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_cells(const float* C, int Hc, int ldc, int FH, int FW, int H, int W,
                         const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
                         hipStream_t s) {
-    MapView mv{C, Hc, ldc, FH, FW};
-    const int h = cells_halo(p);
-    const int hs = (h + 3) / 4 * 4;
-#define X(HS) \
-    if (hs <= HS) return launch_cells_hs<(HS > 0 ? HS : 4)>(mv, FH, FW, H, W, p, Dm, Sm, maxslot, s);
-    BIMP_CELL_HALOS(X)
-#undef X
-    return hipErrorInvalidValue;
+    const CellScale one{C, Hc, ldc, FH, FW, H, W, p, Dm, Sm, maxslot};
+    return launch_cells(&one, 1, s);
 }
 
 hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s) {

@@ -115,6 +115,18 @@ hipError_t launch_inv_rows(int FW, int FH, const ScaleSet& ss, const float2* T, 
 hipError_t launch_cells(const float* C, int Hc, int ldc, int FH, int FW, int H, int W,
                         const CellParams& p, float* Dm, float* Sm, unsigned* maxslot,
                         hipStream_t s);
+// End-stop/inhibition of up to kCellMaxScales scales in one launch (the arguments of
+// launch_cells per scale).
+constexpr int kCellMaxScales = 8;
+struct CellScale {
+    const float* C;
+    int Hc, ldc, FH, FW, H, W;
+    CellParams p;
+    float* Dm;
+    float* Sm;
+    unsigned* maxslot;
+};
+hipError_t launch_cells(const CellScale* scales, int n, hipStream_t s);
 hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s);
 hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_blocks,
                             const unsigned* maxslots, double thresh, int* block_counts,
