@@ -118,6 +118,7 @@ void free_all(bimp_ctx* c) {
     (void)hipFree(c->maxslots);
     (void)hipFree(c->counter);
     (void)hipFree(c->blockCounts);
+    (void)hipFree(c->blockMasks);
     (void)hipFree(c->nmsTable);
     (void)hipFree(c->kpBuf);
     if (c->hostIn) (void)hipHostFree(c->hostIn);
@@ -408,17 +409,17 @@ int run_keypoints(bimp_ctx* c, void* out, int cap, int matrix_ld, hipStream_t s)
     {
         LaunchScope ls(c, s, F_NMS_COUNT, dbytes);
         HIPCHK(c, bimpk::launch_nms_count(c->nmsTable, (int)c->sc.size(), c->totalBlocks,
-                                          c->maxslots, c->opt.nms_threshold, c->blockCounts, s));
+                                          c->maxslots, c->opt.nms_threshold, c->blockCounts,
+                                          c->blockMasks, s));
     }
     {
         LaunchScope ls(c, s, F_NMS_SCAN, 8.0 * c->totalBlocks);
         HIPCHK(c, bimpk::launch_nms_scan(c->blockCounts, c->totalBlocks, c->counter, s));
     }
     {
-        LaunchScope ls(c, s, F_NMS_WRITE, dbytes);
+        LaunchScope ls(c, s, F_NMS_WRITE, 4.0 * bimpk::kNmsThreads * c->totalBlocks);
         HIPCHK(c, bimpk::launch_nms_write(c->nmsTable, (int)c->sc.size(), c->totalBlocks,
-                                          c->maxslots, c->opt.nms_threshold, c->blockCounts,
-                                          out, cap, matrix_ld, s));
+                                          c->blockCounts, c->blockMasks, out, cap, matrix_ld, s));
     }
     return BIMP_OK;
 }
@@ -940,6 +941,7 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
     }
     c->totalBlocks = blocks;
     if (dalloc(&c->nmsTable, tab.size()) != hipSuccess || dalloc(&c->blockCounts, blocks + 1) != hipSuccess ||
+        dalloc(&c->blockMasks, (size_t)blocks * bimpk::kNmsThreads) != hipSuccess ||
         dalloc(&c->maxslots, c->sc.size()) != hipSuccess || dalloc(&c->counter, 1) != hipSuccess)
         return bail(BIMP_E_ALLOC, "nms buffers");
     if (hipMemcpy(c->nmsTable, tab.data(), tab.size() * sizeof(bimpk::NmsScale),

@@ -83,6 +83,7 @@ struct bimp_ctx {
     unsigned* maxslots = nullptr;
     int* counter = nullptr;
     int* blockCounts = nullptr;
+    unsigned* blockMasks = nullptr;  // NMS hit masks, totalBlocks x kNmsThreads
     int totalBlocks = 0;
     bimpk::NmsScale* nmsTable = nullptr;
     void* kpBuf = nullptr;

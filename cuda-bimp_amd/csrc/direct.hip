@@ -37,9 +37,6 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-// Global-memory pointers: loads and stores through them are global_* instructions, counted
-// by vmcnt only (through a generic pointer they are flat_* and also hold up LDS waits).
-#define BIMP_GLOBAL __attribute__((address_space(1)))
 
 template <class T>
 struct Mfma;

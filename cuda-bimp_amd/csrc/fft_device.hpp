@@ -269,6 +269,10 @@ __device__ __forceinline__ void stage_sync() {
 struct LdsTw {
     const __attribute__((address_space(3))) float4* p;
 };
+// Twiddles read from global memory (L1/L2 hits): a global-address-space pointer, so that the
+// loads are global_* (vmcnt only) and not flat_*, which would also hold up the LDS waits of
+// the exchanges.
+typedef const __attribute__((address_space(1))) float2* GlobalTw;
 
 template <int N, bool INV, int Ns, int R, class TW>
 __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane, TW tw) {
@@ -285,9 +289,14 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
                 for (int q = 0; q < P / 2; ++q)
                     t[q] = *(const float4*)(tw.p + q * Ns + k);  // generic: address space inferred
             } else {
-                const float4* row = reinterpret_cast<const float4*>(tw + k * P);
+                typedef float f4v __attribute__((ext_vector_type(4)));
+                const __attribute__((address_space(1))) f4v* row =
+                    (const __attribute__((address_space(1))) f4v*)(tw + k * P);
 #pragma unroll
-                for (int q = 0; q < P / 2; ++q) t[q] = row[q];
+                for (int q = 0; q < P / 2; ++q) {
+                    const f4v x = row[q];
+                    t[q] = make_float4(x.x, x.y, x.z, x.w);
+                }
             }
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -448,8 +457,9 @@ __device__ __forceinline__ void tw_to_lds(const float2* tw, float4* dst, int tid
 template <int N, bool INV, int VMAX = 16>
 __device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane,
                                     const float2* tw) {
-    asm volatile("" : "+s"(tw));
-    Stages<N, INV, 1, typename FFTCfg<N>::Rad, VMAX>::run(v, lds, lane, tw);
+    GlobalTw g = (GlobalTw)tw;
+    asm volatile("" : "+s"(g));
+    Stages<N, INV, 1, typename FFTCfg<N>::Rad, VMAX>::run(v, lds, lane, g);
 }
 template <int N, bool INV, int VMAX = 16>
 __device__ __forceinline__ void fft(float2 (&v)[FFTCfg<N>::E], float* lds, int lane, LdsTw tw) {

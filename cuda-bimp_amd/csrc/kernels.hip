@@ -825,13 +825,13 @@ __device__ __forceinline__ int find_scale(const NmsScale* sc, int nscales, int b
 // identifyKeypoints_kernel predicate (cuda_bimp.cuh:100-108): exclusive border `off`,
 // D - max8(D) > 0 (dilate with the centre removed, cuda_bimp.cu:73-85, :733), D > thresh.
 // Evaluated as D > thresh first (one load, rarely true), then the rest:
-__device__ __forceinline__ bool nms_rest(const float* D, int H, int W, int off, float v, int y,
-                                         int x) {
+__device__ __forceinline__ bool nms_rest(const BIMP_GLOBAL float* D, int H, int W, int off,
+                                         float v, int y, int x) {
 #pragma clang fp contract(off)
     if (!(x > off && x < W - off && y > off && y < H - off)) return false;
-    const float* p = D + (size_t)y * W;
-    const float* pu = p - W;
-    const float* pd = p + W;
+    const BIMP_GLOBAL float* p = D + (size_t)y * W;
+    const BIMP_GLOBAL float* pu = p - W;
+    const BIMP_GLOBAL float* pd = p + W;
     float nb = fmaxf(pu[x - 1], pu[x]);
     nb = fmaxf(nb, pu[x + 1]);
     nb = fmaxf(nb, p[x - 1]);
@@ -847,11 +847,12 @@ __device__ __forceinline__ bool nms_rest(const float* D, int H, int W, int off, 
 // computed for the few above the threshold.
 __device__ __forceinline__ unsigned nms_mask(const NmsScale& d, float thresh, int base) {
     const int A = d.H * d.W;
+    const BIMP_GLOBAL float* D = (const BIMP_GLOBAL float*)d.D;
     float v[kNmsPerThread];
 #pragma unroll
     for (int i = 0; i < kNmsPerThread; ++i) {
         const int pix = base + i * kNmsThreads + (int)threadIdx.x;
-        v[i] = pix < A ? d.D[pix] : 0.f;
+        v[i] = pix < A ? D[pix] : 0.f;
     }
     unsigned mask = 0;
 #pragma unroll
@@ -859,7 +860,7 @@ __device__ __forceinline__ unsigned nms_mask(const NmsScale& d, float thresh, in
         const int pix = base + i * kNmsThreads + (int)threadIdx.x;
         if (pix < A && v[i] > thresh) {
             const int y = pix / d.W, x = pix - y * d.W;
-            if (nms_rest(d.D, d.H, d.W, d.off, v[i], y, x)) mask |= 1u << i;
+            if (nms_rest(D, d.H, d.W, d.off, v[i], y, x)) mask |= 1u << i;
         }
     }
     return mask;
@@ -873,13 +874,16 @@ __device__ __forceinline__ float scale_thresh(const unsigned* maxslots, int slot
 __global__ void __launch_bounds__(kNmsThreads) k_nms_count(const NmsScale* __restrict__ sc,
                                                            int nscales,
                                                            const unsigned* __restrict__ maxslots,
-                                                           double thr, int* __restrict__ counts) {
+                                                           double thr, int* __restrict__ counts,
+                                                           unsigned* __restrict__ masks) {
     const int blk = blockIdx.x;
     const int s = find_scale(sc, nscales, blk);
     const NmsScale d = sc[s];
     const float thresh = scale_thresh(maxslots, d.slot, thr);
     const int base = (blk - d.block0) * kNmsBlockPixels;
-    int c = __popc(nms_mask(d, thresh, base));
+    const unsigned mask = nms_mask(d, thresh, base);
+    ((BIMP_GLOBAL unsigned*)masks)[(size_t)blk * kNmsThreads + threadIdx.x] = mask;
+    int c = __popc(mask);
 #pragma unroll
     for (int sh = 32; sh >= 1; sh >>= 1) c += __shfl_xor(c, sh, 64);
     __shared__ int wc[kNmsThreads / 64];
@@ -922,12 +926,11 @@ __global__ void __launch_bounds__(1024) k_nms_scan(int* __restrict__ counts, int
     }
 }
 
-// Re-evaluates the predicate and writes each keypoint at its row-major rank.  matrix_ld > 0
+// Writes each keypoint of k_nms_count's hit masks at its row-major rank.  matrix_ld > 0
 // selects the 7-row SURF_CUDA layout (cuda_bimp.cuh:132-140) instead of keypoint structs.
 __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
-        const NmsScale* __restrict__ sc, int nscales, const unsigned* __restrict__ maxslots,
-        double thr, const int* __restrict__ offsets, void* __restrict__ out, int cap,
-        int matrix_ld) {
+        const NmsScale* __restrict__ sc, int nscales, const int* __restrict__ offsets,
+        const unsigned* __restrict__ masks, void* __restrict__ out, int cap, int matrix_ld) {
 #pragma clang fp contract(off)
     const int blk = blockIdx.x;
     const int s = find_scale(sc, nscales, blk);
@@ -936,14 +939,14 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
     // a block without keypoints (its count from the scan), or whose first rank is past the
     // capacity, writes nothing
     if (offsets[blk + 1] == start || start >= cap) return;
-    const float thresh = scale_thresh(maxslots, d.slot, thr);
     const int base = (blk - d.block0) * kNmsBlockPixels;
     constexpr int NW = kNmsThreads / 64;
     // rank of pixel (i, thread) = keypoints of earlier iterations, then of earlier waves in
     // iteration i, then of earlier lanes: the row-major order of the pixels
     __shared__ int wc[kNmsPerThread * NW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned mask = nms_mask(d, thresh, base);
+    const unsigned mask =
+        ((const BIMP_GLOBAL unsigned*)masks)[(size_t)blk * kNmsThreads + threadIdx.x];
     unsigned rank_lo[kNmsPerThread];
 #pragma unroll
     for (int i = 0; i < kNmsPerThread; ++i) {
@@ -968,7 +971,7 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
             const int y = pix / d.W, x = pix - y * d.W;
             const int idx = wc[i * NW + wave] + (int)rank_lo[i];
             if (idx < cap) {
-                const float* p = d.D + (size_t)y * d.W;
+                const BIMP_GLOBAL float* p = (const BIMP_GLOBAL float*)d.D + (size_t)y * d.W;
                 const float xv1 = p[x - 1], xv2 = p[x], xv3 = p[x + 1];
                 const float yv1 = p[x - d.W], yv3 = p[x + d.W];
                 const float denom_x = xv1 + xv3 - 2.f * xv2;
@@ -1381,9 +1384,9 @@ hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s) {
 
 hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_blocks,
                             const unsigned* maxslots, double thresh, int* block_counts,
-                            hipStream_t s) {
+                            unsigned* masks, hipStream_t s) {
     k_nms_count<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, maxslots, thresh,
-                                                     block_counts);
+                                                     block_counts, masks);
     return hipGetLastError();
 }
 
@@ -1393,10 +1396,10 @@ hipError_t launch_nms_scan(int* block_counts, int total_blocks, int* counter, hi
 }
 
 hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_blocks,
-                            const unsigned* maxslots, double thresh, const int* block_offsets,
-                            void* out, int cap, int matrix_ld, hipStream_t s) {
-    k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, maxslots, thresh,
-                                                     block_offsets, out, cap, matrix_ld);
+                            const int* block_offsets, const unsigned* masks, void* out, int cap,
+                            int matrix_ld, hipStream_t s) {
+    k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, block_offsets, masks, out,
+                                                     cap, matrix_ld);
     return hipGetLastError();
 }
 

@@ -28,6 +28,11 @@ struct CellParams {
     int ra[kNumOri], rb[kNumOri];  // radial half offsets
 };
 
+// Global-memory pointer qualifier: loads and stores through such pointers are global_*
+// instructions, counted by vmcnt only (through a generic pointer they are flat_* and also
+// hold up every LDS wait).
+#define BIMP_GLOBAL __attribute__((address_space(1)))
+
 // Per-scale descriptor for the NMS kernels (device table).
 struct NmsScale {
     const float* D;     // dataH x dataW response map
@@ -128,13 +133,15 @@ struct CellScale {
 };
 hipError_t launch_cells(const CellScale* scales, int n, hipStream_t s);
 hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s);
+// k_nms_count also stores each thread's hit mask (total_blocks x kNmsThreads words) for
+// k_nms_write, which then reads the response maps only around the hits.
 hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_blocks,
                             const unsigned* maxslots, double thresh, int* block_counts,
-                            hipStream_t s);
+                            unsigned* masks, hipStream_t s);
 hipError_t launch_nms_scan(int* block_counts, int total_blocks, int* counter, hipStream_t s);
 hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_blocks,
-                            const unsigned* maxslots, double thresh, const int* block_offsets,
-                            void* out, int cap, int matrix_ld, hipStream_t s);
+                            const int* block_offsets, const unsigned* masks, void* out, int cap,
+                            int matrix_ld, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_bgr_to_gray(const uint8_t* bgr, size_t bpitch, int w, int h, uint8_t* gray,
                               size_t gpitch, hipStream_t s);
