@@ -844,7 +844,9 @@ __device__ __forceinline__ bool nms_rest(const BIMP_GLOBAL float* D, int H, int 
 
 // Hit mask of a thread's kNmsPerThread pixels (bit i: pixel base + i*kNmsThreads + tid).
 // All the pixels' loads go out first; the row/column split and the neighbourhood are only
-// computed for the few above the threshold.
+// computed for the few above the threshold (about 2% of the pixels), each lane walking its
+// own candidates, so that a wavefront takes as many neighbourhood rounds as its busiest lane
+// has candidates rather than one per pixel slot that any lane needs.
 __device__ __forceinline__ unsigned nms_mask(const NmsScale& d, float thresh, int base) {
     const int A = d.H * d.W;
     const BIMP_GLOBAL float* D = (const BIMP_GLOBAL float*)d.D;
@@ -854,14 +856,19 @@ __device__ __forceinline__ unsigned nms_mask(const NmsScale& d, float thresh, in
         const int pix = base + i * kNmsThreads + (int)threadIdx.x;
         v[i] = pix < A ? D[pix] : 0.f;
     }
-    unsigned mask = 0;
+    unsigned cand = 0;
 #pragma unroll
     for (int i = 0; i < kNmsPerThread; ++i) {
         const int pix = base + i * kNmsThreads + (int)threadIdx.x;
-        if (pix < A && v[i] > thresh) {
-            const int y = pix / d.W, x = pix - y * d.W;
-            if (nms_rest(D, d.H, d.W, d.off, v[i], y, x)) mask |= 1u << i;
-        }
+        if (pix < A && v[i] > thresh) cand |= 1u << i;
+    }
+    unsigned mask = 0;
+    while (cand) {
+        const int i = __builtin_ctz(cand);
+        cand &= cand - 1u;
+        const int pix = base + i * kNmsThreads + (int)threadIdx.x;
+        const int y = pix / d.W, x = pix - y * d.W;
+        if (nms_rest(D, d.H, d.W, d.off, D[pix], y, x)) mask |= 1u << i;
     }
     return mask;
 }
