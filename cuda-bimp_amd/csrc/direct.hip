@@ -105,21 +105,25 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
     // Stage rows y0-k .. and columns x0-k .. of Pd as two copies, the second shifted by one
     // sample (tile sample c at position c - s of copy s), so that every lane's 8-sample window
     // starts on a 4-byte boundary in one of them.  Positions past the image window hold zero;
-    // they only meet zero taps.  Wave w takes rows w, w + NW, ..., lane l columns l, l + 64,
-    // ...; the byte loads of kStageRows rows are all issued before any is written, so that
-    // their latencies overlap instead of adding up row by row.
+    // they only meet zero taps.  Wave w takes rows w, w + NW, ..., lane l the four columns
+    // 4l .. 4l+3, written as one 8-byte store per copy (the shifted copy's fourth sample is
+    // the next lane's first); the byte loads of kStageRows rows are all issued before any is
+    // written, so that their latencies overlap instead of adding up row by row.
     const bool inside = y0 - k >= 0 && y0 + TY + k <= ds.H && x0 - k >= 0 && x0 + TX + k <= ds.W;
     {
-        constexpr int kStageRows = 4, kCols = kDirMaxLwp / 64;
-        // source column of staged column lane + 64 i (-1: zero), the same for every row
-        int gxs[kCols];
+        constexpr int kStageRows = 4;
+        static_assert(kDirMaxLwp == 4 * 64, "one group of four staged columns per lane");
+        typedef typename Mfma<T>::v4 v4;
+        // source columns of staged columns c0 .. c0+3 (-1: zero), the same for every row
+        const int c0 = 4 * lane;
+        int gxs[4];
 #pragma unroll
-        for (int i = 0; i < kCols; ++i) {
-            const int c = lane + 64 * i;
-            gxs[i] = c >= TWi ? -1 : (inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k));
+        for (int j = 0; j < 4; ++j) {
+            const int c = c0 + j;
+            gxs[j] = c >= TWi ? -1 : (inside ? x0 - k + c : pad_index(x0 - k + c, ds.W, ds.FW, k));
         }
         for (int r0 = wave; r0 < ds.LH; r0 += kStageRows * NW) {
-            float v[kStageRows][kCols];
+            float v[kStageRows][4];
 #pragma unroll
             for (int q = 0; q < kStageRows; ++q) {
                 const int r = r0 + q * NW;
@@ -128,21 +132,22 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
                                       : (inside ? y0 - k + r : pad_index(y0 - k + r, ds.H, ds.FH, k));
                 const BIMP_GLOBAL uint8_t* row = img + (size_t)gy * pitch;
 #pragma unroll
-                for (int i = 0; i < kCols; ++i)
-                    v[q][i] = (rowin && gxs[i] >= 0) ? (float)row[gxs[i]] : 0.f;
+                for (int j = 0; j < 4; ++j)
+                    v[q][j] = (rowin && gxs[j] >= 0) ? (float)row[gxs[j]] : 0.f;
             }
 #pragma unroll
             for (int q = 0; q < kStageRows; ++q) {
                 const int r = r0 + q * NW;
-                if (r >= ds.LH) break;
-                T* dst = copies + r * LWP;
-#pragma unroll
-                for (int i = 0; i < kCols; ++i) {
-                    const int c = lane + 64 * i;
-                    if (c >= LWP) break;
-                    const T tv = T(v[q][i]);
-                    dst[c] = tv;
-                    if (c >= 1) dst[csize + c - 1] = tv;
+                if (r >= ds.LH) break;  // uniform
+                // sample c0 + 4: the next lane's first (column 256 and beyond: zero)
+                const float next = __shfl_down(v[q][0], 1, 64);
+                const float nx = lane == 63 ? 0.f : next;
+                if (c0 < LWP) {  // LWP is a multiple of 16: the lane's four positions fit
+                    T* dst = copies + r * LWP + c0;
+                    const v4 a = {T(v[q][0]), T(v[q][1]), T(v[q][2]), T(v[q][3])};
+                    const v4 b = {T(v[q][1]), T(v[q][2]), T(v[q][3]), T(nx)};
+                    *reinterpret_cast<v4*>(dst) = a;          // copy 0: position c = sample c
+                    *reinterpret_cast<v4*>(dst + csize) = b;  // copy 1: position c = sample c+1
                 }
             }
         }
@@ -346,10 +351,10 @@ size_t direct_geometry(DirectScale& d) {
     // row pitch 16 (mod 32) elements: the 4 tap rows of a read start 8 x (odd) banks apart
     d.lwp = tws + ((16 - tws % 32) + 32) % 32;
     d.vsp = twi;
-    // copy stride 64 (mod 128) elements: the two copies sit 32 banks apart, so the 64 lanes
-    // of a 4-byte read (4 tap rows x 2 copies x 8 consecutive words) hit 64 distinct banks
+    // copy stride 32 (mod 64) elements: the two copies sit 16 banks apart, so each 16-lane
+    // group of a fragment read (8 consecutive words in each copy) hits distinct banks
     const int cs = d.LH * d.lwp;
-    d.cstride = cs + ((64 - cs % 128) + 128) % 128;
+    d.cstride = cs + ((32 - cs % 64) + 64) % 64;
     return (size_t)2 * d.cstride * 2 + (size_t)kDirTY * d.vsp * sizeof(float);
 }
 
