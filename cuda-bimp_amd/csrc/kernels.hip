@@ -397,7 +397,7 @@ struct InvRowCfg {
     static constexpr int THREADS = 64 * WPB;
     static constexpr size_t MAIN = (size_t)WPB * fft_lds_floats<FW>() * sizeof(float);
     static constexpr size_t LDS = MAIN + tw_lds_bytes<FW>();
-    static constexpr bool PERSISTENT = FFTCfg<FW>::TPV == 64 && FW <= 2048;
+    static constexpr bool PERSISTENT = FFTCfg<FW>::TPV == 64;
 };
 
 template <int FW>
