@@ -116,9 +116,12 @@ struct ColCfg {
     static constexpr int TPV = FFTCfg<N>::TPV;
     static constexpr int CB = TPV == 64 ? (N <= 2048 ? 8 : 4) : (512 / TPV > 16 ? 16 : 512 / TPV);
     static constexpr int THREADS = CB * TPV;
+    // The forward column pass stages its columns in LDS in two halves of N/2 rows when whole
+    // columns would take more than 64 KB, so that several workgroups share a CU.
+    static constexpr int HALVES = (size_t)CB * (N + 2) * sizeof(float2) > 64u * 1024u ? 2 : 1;
     // column stride (complex): 2 (mod 16) puts the 8 columns of a 16-lane access 4 banks
     // apart, so staging writes (column index fastest) do not collide
-    static constexpr int S = N + 2;
+    static constexpr int S = N / HALVES + 2;
 };
 
 // Column FFT of the row spectra R (H rows, ldr complex per row) with clamp padding in y:
@@ -146,11 +149,21 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
     const float2* R = Rbase + (size_t)sidx * rstride;
     float2* DT = ss.DT[sidx];
     const int kx0 = blk * CC::CB;
-    {
-        // a thread stages column b of the block, rows y0 + i*YS; eight loads go out before
-        // their LDS stores
-        static_assert(CC::THREADS % CC::CB == 0 && (FH * CC::CB) % CC::THREADS == 0, "layout");
-        constexpr int YS = CC::THREADS / CC::CB, IT = FH * CC::CB / CC::THREADS;
+    const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
+    float2* my = lds2 + (size_t)g * CC::S;
+    float2 v[C::E];
+    constexpr int HN = FH / CC::HALVES;  // rows staged per half
+    // register slot e holds row in_index(e) in [(e % R) N/R, (e % R + 1) N/R), R the first
+    // radix: with R even its half is known at compile time
+    constexpr int R1 = RadInfo<typename C::Rad>::first;
+    static_assert(CC::HALVES == 1 || R1 % 2 == 0, "halves follow the first radix");
+#pragma unroll
+    for (int h = 0; h < CC::HALVES; ++h) {
+        if (h > 0) __syncthreads();  // every wavefront has read the previous half
+        // a thread stages column b of the block, rows h*HN + y0 + i*YS; eight loads go out
+        // before their LDS stores
+        static_assert(CC::THREADS % CC::CB == 0 && (HN * CC::CB) % CC::THREADS == 0, "layout");
+        constexpr int YS = CC::THREADS / CC::CB, IT = HN * CC::CB / CC::THREADS;
         const int b = threadIdx.x % CC::CB, y0 = threadIdx.x / CC::CB;
         const int kx = kx0 + b;
         const bool in = kx < ncols;
@@ -159,7 +172,7 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
             float2 val[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int y = y0 + (i0 + j) * YS;
+                const int y = h * HN + y0 + (i0 + j) * YS;
                 const int ry = y < H ? y : (y < H + kY ? H - 1 : 0);
                 val[j] = (i0 + j < IT && in) ? R[(size_t)ry * ldr + kx] : make_float2(0.f, 0.f);
             }
@@ -167,13 +180,12 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
             for (int j = 0; j < 8; ++j)
                 if (i0 + j < IT) lds2[b * CC::S + y0 + (i0 + j) * YS] = val[j];
         }
-    }
-    __syncthreads();
-    const int g = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
-    float2* my = lds2 + (size_t)g * CC::S;
-    float2 v[C::E];
+        __syncthreads();
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) v[e] = my[in_index<FH>(e, lane)];
+        for (int e = 0; e < C::E; ++e)
+            if (CC::HALVES == 1 || ((e % R1) >= R1 / 2) == (h == 1))
+                v[e] = my[in_index<FH>(e, lane) - h * HN];
+    }
     stage_sync();
     fft<FH, false>(v, reinterpret_cast<float*>(my), lane, tw);
     const int kx = kx0 + g;
@@ -208,7 +220,12 @@ struct InvColCfg {
     // The persistent workgroup keeps the twiddle table in LDS (after the exchange/staging
     // region) when that still leaves room for two workgroups per CU: LDS reads instead of
     // cache round trips inside each stage.
-    static constexpr bool LDS_TW = MAIN + tw_lds_bytes<FH>() <= 80u * 1024u;
+    // 4-wavefront workgroups (CB = 4) need three per CU for enough wavefronts: the twiddles
+    // then stay in global memory (L1/L2 hits) when keeping them in LDS would leave room for
+    // only two (2560 points: 516 -> 422 us per launch at 3840x2160)
+    static constexpr bool TW_COSTS_A_WG = CB == 4 && 3 * MAIN <= kMaxLds &&
+                                          3 * (MAIN + tw_lds_bytes<FH>()) > kMaxLds;
+    static constexpr bool LDS_TW = MAIN + tw_lds_bytes<FH>() <= 80u * 1024u && !TW_COSTS_A_WG;
     static constexpr size_t LDS = MAIN + (LDS_TW ? tw_lds_bytes<FH>() : 0);
 };
 
