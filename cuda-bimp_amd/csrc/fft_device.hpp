@@ -168,9 +168,16 @@ __device__ __forceinline__ void dft16(float2* v) {
     }
 }
 
+template <int N1, int N2, bool INV>
+__device__ __forceinline__ void dft_pfa(float2* v);
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2* v) {
-    if constexpr (R == 2) dft2<INV>(v);
+    if constexpr (R == 24) dft_pfa<8, 3, INV>(v);
+    else if constexpr (R == 40) dft_pfa<8, 5, INV>(v);
+    else if constexpr (R == 48) dft_pfa<16, 3, INV>(v);
+    else if constexpr (R == 56) dft_pfa<8, 7, INV>(v);
+    else if constexpr (R == 2) dft2<INV>(v);
     else if constexpr (R == 3) dft3<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v);
     else if constexpr (R == 5) dft5<INV>(v);
@@ -178,6 +185,43 @@ __device__ __forceinline__ void dft(float2* v) {
     else if constexpr (R == 8) dft8<INV>(v);
     else if constexpr (R == 16) dft16<INV>(v);
     else static_assert(R == 2, "unsupported radix");
+}
+
+// q with a*q = 1 (mod m), a and m coprime
+__host__ __device__ constexpr int inv_mod(int a, int m) {
+    for (int q = 1; q < m; ++q)
+        if (a * q % m == 1) return q;
+    return 1;
+}
+
+// Prime-factor (Good-Thomas) DFT of length N = N1*N2, gcd(N1, N2) = 1: with input index
+// n = (N2 n1 + N1 n2) mod N and output index k = (N2 q1 k1 + N1 q2 k2) mod N, where
+// q1 = N2^-1 mod N1 and q2 = N1^-1 mod N2, w_N^(nk) = w_N1^(n1 k1) w_N2^(n2 k2), so N2
+// length-N1 DFTs followed by N1 length-N2 DFTs give the transform with no twiddles between
+// them.  All indices are compile-time: the permutations are register renamings.
+template <int N1, int N2, bool INV>
+__device__ __forceinline__ void dft_pfa(float2* v) {
+    constexpr int N = N1 * N2;
+    constexpr int q1 = inv_mod(N2 % N1, N1), q2 = inv_mod(N1 % N2, N2);
+    float2 y[N];  // y[k1 * N2 + n2]
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+        float2 a[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) a[n1] = v[(N2 * n1 + N1 * n2) % N];
+        dft<N1, INV>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) y[k1 * N2 + n2] = a[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        float2 b[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) b[n2] = y[k1 * N2 + n2];
+        dft<N2, INV>(b);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[(N2 * q1 * k1 + N1 * q2 * k2) % N] = b[k2];
+    }
 }
 
 // ---------------------------------------------------------------- plans
@@ -202,11 +246,13 @@ BIMP_FFT_CFG(128, 16, 8, 8, 2)
 BIMP_FFT_CFG(256, 32, 8, 8, 4)
 BIMP_FFT_CFG(512, 64, 8, 8, 8)
 BIMP_FFT_CFG(1024, 64, 16, 16, 4)
-BIMP_FFT_CFG(1536, 64, 8, 8, 8, 3)
+// The lengths with an odd factor take it in a prime-factor first stage (24 = 8*3, 40 = 8*5,
+// 48 = 16*3, 56 = 8*7): three stages and two LDS exchanges instead of four and three.
+BIMP_FFT_CFG(1536, 64, 24, 8, 8)
 BIMP_FFT_CFG(2048, 64, 16, 16, 8)
-BIMP_FFT_CFG(2560, 64, 8, 8, 8, 5)
-BIMP_FFT_CFG(3072, 64, 16, 16, 4, 3)
-BIMP_FFT_CFG(3584, 64, 8, 8, 8, 7)
+BIMP_FFT_CFG(2560, 64, 40, 8, 8)
+BIMP_FFT_CFG(3072, 64, 48, 8, 8)
+BIMP_FFT_CFG(3584, 64, 56, 8, 8)
 BIMP_FFT_CFG(4096, 64, 16, 16, 16)
 #undef BIMP_FFT_CFG
 
@@ -314,6 +360,7 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
 // the first stage lane j holds its R outputs j*R .. j*R+R-1.  When R <= VMAX (a parameter of
 // the transform) they leave as R/4 float4 writes: consecutive lanes' 16-byte writes 4R bytes
 // apart, one instruction for four scalar writes and no per-element address arithmetic.
+// The prime-factor radices (R > 16) always take the float4 form.
 // Otherwise each is a scalar write whose address XOR-swizzles the bank bits of its 32-word row
 // with the row index (a bijection within the row: it spreads the stride-R writes over all
 // banks and keeps the reads of 32 consecutive elements conflict-free, at no extra LDS).  The
@@ -321,7 +368,7 @@ __device__ __forceinline__ void butterflies(float2 (&v)[FFTCfg<N>::E], int lane,
 // inverse row passes, which therefore keep the swizzle for R = 16.  Later exchanges write runs
 // of >= R consecutive lanes and keep the plain layout.
 template <int Ns, int R, int VMAX>
-inline constexpr bool kVecFirst = Ns == 1 && R % 4 == 0 && R <= VMAX;
+inline constexpr bool kVecFirst = Ns == 1 && R % 4 == 0 && (R <= VMAX || R > 16);
 
 template <int N>
 __host__ __device__ constexpr int fft_lds_floats() { return N; }

@@ -213,9 +213,20 @@ struct InvColCfg {
     // lane + TPV*(e/RL) + (e%RL)*BR, so block membership is known at compile time
     static constexpr int RL = RadInfo<typename FFTCfg<FH>::Rad>::last;
     static constexpr int BR = FH / RL;
-    static constexpr int SS = BR + 2;  // staging stride (float2); 2 (mod 16): conflict-free reads
-    static constexpr size_t STAGE = (size_t)CB * SS * sizeof(float2);
     static constexpr size_t EXCH = (size_t)CB * fft_lds_floats<FH>() * sizeof(float);
+    // GB consecutive blocks (contiguous rows) leave per staging round, each round costing two
+    // workgroup barriers: the most that divide RL and stay within half a column, so the
+    // staging region is the exchange region plus the 2-element pad per column (3840x2160:
+    // two rounds instead of four cut k_inv_cols<2560> from 413 to 292 us per launch)
+    static constexpr int pick_gb() {
+        for (int g = RL; g > 1; --g)
+            if (RL % g == 0 && g * BR <= FH / 2) return g;
+        return 1;
+    }
+    static constexpr int GB = pick_gb();
+    static constexpr int BRG = GB * BR;
+    static constexpr int SS = BRG + 2;  // staging stride (float2); 2 (mod 16): conflict-free reads
+    static constexpr size_t STAGE = (size_t)CB * SS * sizeof(float2);
     static constexpr size_t MAIN = STAGE > EXCH ? STAGE : EXCH;
     // The persistent workgroup keeps the twiddle table in LDS (after the exchange/staging
     // region) when that still leaves room for two workgroups per CU: LDS reads instead of
@@ -312,33 +323,37 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_inv_cols(
         // of CB); where a round takes 8 or more such rows, four LDS reads go out before their
         // four stores instead of one read waiting before each store
         static_assert(IC::THREADS % CB == 0, "whole rows of the block per pass");
-        constexpr int YS = IC::THREADS / CB, IT = (IC::BR + YS - 1) / YS;
-        constexpr bool BATCH = IT >= 8 && IT % 4 == 0 && IC::BR % YS == 0;
+        constexpr int YS = IC::THREADS / CB, IT = (IC::BRG + YS - 1) / YS;
+        constexpr int J = IT % 4 == 0 ? 4 : IT % 3 == 0 ? 3 : IT % 2 == 0 ? 2 : 1;
+        constexpr bool BATCH = IT >= 6 && J > 1 && IC::BRG % YS == 0;
         const int b = threadIdx.x % CB, yy0 = threadIdx.x / CB;
         const int ks = partner ? k0 + CB - b : k0 + b;  // stored column
         const bool wcol = partner ? (ks < half) : (ks <= half);
 #pragma unroll
-        for (int rb = 0; rb < IC::RL; ++rb) {
-            const int ylo = rb * IC::BR;
+        for (int rq = 0; rq < IC::RL / IC::GB; ++rq) {
+            const int ylo = rq * IC::BRG;
             if (ylo >= Hc) break;  // uniform
-            const int rows = min(Hc - ylo, IC::BR);
+            const int rows = min(Hc - ylo, IC::BRG);
             __syncthreads();  // transforms and earlier readers of the region are done
 #pragma unroll
             for (int m = 0; m < C::E / IC::RL; ++m)
-                lds2[slot * IC::SS + lane + C::TPV * m] = v[m * IC::RL + rb];
+#pragma unroll
+                for (int gb = 0; gb < IC::GB; ++gb)
+                    lds2[slot * IC::SS + gb * IC::BR + lane + C::TPV * m] =
+                            v[m * IC::RL + rq * IC::GB + gb];
             __syncthreads();
             // T is column-block-major: block base / CB holds rows 0..Hc-1 of its CB columns
             // contiguously, so a unit's rows go out as one contiguous run
             float2* const blk = To + (size_t)(base / CB) * Hc * CB;
             if constexpr (BATCH) {
 #pragma unroll 1
-                for (int i0 = 0; i0 < IT; i0 += 4) {
-                    float2 val[4];
+                for (int i0 = 0; i0 < IT; i0 += J) {
+                    float2 val[J];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) val[j] = lds2[b * IC::SS + yy0 + (i0 + j) * YS];
+                    for (int j = 0; j < J; ++j) val[j] = lds2[b * IC::SS + yy0 + (i0 + j) * YS];
                     float2* dst = blk + (size_t)(ylo + yy0 + i0 * YS) * CB + b;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < J; ++j)
                         if (wcol && yy0 + (i0 + j) * YS < rows) dst[(size_t)j * YS * CB] = val[j];
                 }
             } else {

@@ -203,7 +203,7 @@ def test_repeat_frames_identical(B):
 @pytest.mark.parametrize("w,h,fw,fh", [(2600, 200, 3072, 256), (3100, 96, 3584, 128),
                                        (48, 2600, 64, 3072), (40, 3100, 64, 3584)])
 def test_transform_sizes_3072_3584(B, w, h, fw, fh):
-    """Sides that snap to 3072 (16*16*4*3) and 3584 (8*8*8*7) in both directions."""
+    """Sides that snap to 3072 (48*8*8, 48 = 16*3) and 3584 (56*8*8, 56 = 8*7) in both directions."""
     lam = O.BASE_BANK[:2]
     cs, orc = _pair(B, w, h, lam)
     assert (cs.ks[0].fft_w, cs.ks[0].fft_h) == (fw, fh)
