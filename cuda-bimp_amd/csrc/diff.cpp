@@ -1,10 +1,14 @@
 // diff.cpp -- host side of the differentiation form (C ABI bimp_diff_*, include/bimp_hip.h).
 //
 // A diff context holds, per scale, the bank spectra Qhat_o of the current Gabor parameters
-// (sigma ratio, gamma) and the tap derivatives dK/dtheta; per frame it recomputes the complex
-// responses instead of storing them (forward and backward each run the filter).  Images are
-// FP32, scaled by 255 * input_scale before filtering, so an image u8/255 gives the maps of
-// the uint8 path up to the pyramid's rounding.
+// (sigma ratio, gamma), their real quadrature form G_o and the tap derivatives dK/dtheta; per
+// frame it recomputes the complex responses instead of storing them (forward and backward
+// each run the filter).  The forward takes the detection path's transforms (two image rows
+// per complex row FFT, half spectra, inverse columns with the Hermitian partner, G_o for
+// o = 0..4 with 5..7 mirrored); the backward needs Z_o, FFT(Pd) and C_o on the whole periodic
+// domain and recomputes them with full complex 2D transforms.  Images are FP32, scaled by
+// 255 * input_scale before filtering, so an image u8/255 gives the maps of the uint8 path up
+// to the pyramid's rounding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,10 +30,11 @@ hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, i
                          hipStream_t s);
 hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int sw, int sh,
                              hipStream_t s);
-hipError_t diff_pad(const float* L, int H, int W, int FH, int FW, int k, float sc, float2* out,
-                    hipStream_t s);
+hipError_t diff_pad(const float* L, int pitch, int H, int W, int FH, int FW, int k, float sc,
+                    float2* out, hipStream_t s);
 hipError_t diff_pad_adj(const float2* g, int H, int W, int FH, int FW, int k, float sc,
-                        float* gL, hipStream_t s);
+                        float* gL, int pitch, hipStream_t s);
+hipError_t diff_q_to_g(const float2* Q, int FH, int FW, float invN, float* G, hipStream_t s);
 hipError_t diff_bank_pad(const float* ke, const float* ko, int kW, int kX, int FH, int FW,
                          float2* out, hipStream_t s);
 hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, float invN,
@@ -55,6 +60,10 @@ struct DScale {
     bimph::ScaleGeom g;
     bimpk::CellParams cp;
     float2* Q = nullptr;   // 8 x FH x FW bank spectra
+    float* G = nullptr;    // 5 x FW x FH quadrature bank Re(Q_o) / (FH*FW), kx-major
+    int Hc = 0, Wc = 0, ldc = 0;  // complex-cell region of the forward (as in detection)
+    float2* DT = nullptr;  // forward: (FW/2+1) x FH half spectrum, column-major
+    float* Cf = nullptr;   // forward: 8 x Hc x ldc complex-cell maps
     float* dK = nullptr;   // 4 x 8 x kW x kW tap derivatives (dKe/dr, dKo/dr, dKe/dg, dKo/dg)
     size_t respOff = 0;    // offset of this scale's D map in a response vector
 };
@@ -75,6 +84,11 @@ struct bimp_diff {
     std::map<int, float2*> tw;
     size_t Pmax = 0;
     float2 *Dhat = nullptr, *Z = nullptr, *acc = nullptr;
+    // forward: scales of one level and transform size share each launch (first, count)
+    std::vector<std::pair<int, int>> groups;
+    float2 *R = nullptr, *T = nullptr;  // forward: row spectra and inverse-column output
+    float* Sf = nullptr;                // forward: single-stopped maps (scratch)
+    size_t rSize = 0, tSize = 0;
     float *C = nullptr, *Wm = nullptr, *gC = nullptr, *S = nullptr, *taps = nullptr;
     unsigned* maxslot = nullptr;
     float* partials = nullptr;
@@ -127,6 +141,7 @@ int set_params(bimp_diff* d, float ratio, float gamma, hipStream_t s) {
         DCHK(d, bimpk::diff_bank_pad(d->taps, d->taps + ke.size(), g.kW, g.kX, g.FH, g.FW, x.Q, s));
         const int st = fft2(d, x.Q, kNumOri, g.FH, g.FW, false, s);
         if (st) return st;
+        DCHK(d, bimpk::diff_q_to_g(x.Q, g.FH, g.FW, 1.0f / ((float)g.FH * (float)g.FW), x.G, s));
         DCHK(d, hipStreamSynchronize(s));
     }
     d->ratio = ratio;
@@ -144,12 +159,14 @@ int pyramid(bimp_diff* d, const float* img, hipStream_t s) {
     return BIMP_OK;
 }
 
-// Forward of one scale: Dhat, Z, C, and the D map into `resp`.
+// Forward of one scale for the backward pass: Dhat, Z, C on the whole FH x FW domain, and
+// the D map into `resp`.
 int scale_forward(bimp_diff* d, DScale& x, float* resp, hipStream_t s) {
     const auto& g = x.g;
     const size_t P = (size_t)g.FH * g.FW;
     const float sc = 255.0f * d->opt.input_scale;
-    DCHK(d, bimpk::diff_pad(d->lev[g.level], g.H, g.W, g.FH, g.FW, g.kX, sc, d->Dhat, s));
+    DCHK(d, bimpk::diff_pad(d->lev[g.level], d->lw[g.level], g.H, g.W, g.FH, g.FW, g.kX, sc,
+                            d->Dhat, s));
     int st = fft2(d, d->Dhat, 1, g.FH, g.FW, false, s);
     if (st) return st;
     DCHK(d, bimpk::diff_spec_mul(d->Dhat, x.Q, d->Z, P, 1.0f / (float)P, s));
@@ -157,6 +174,40 @@ int scale_forward(bimp_diff* d, DScale& x, float* resp, hipStream_t s) {
     DCHK(d, bimpk::diff_mag(d->Z, d->C, (size_t)kNumOri * P, s));
     DCHK(d, bimpk::launch_cells(d->C, g.FH, g.FW, g.FH, g.FW, g.H, g.W, x.cp, resp, d->S,
                                 d->maxslot, s));
+    return BIMP_OK;
+}
+
+// The D maps of scales first..first+n-1 (one pyramid level, one transform size) into the
+// response vector `resp` through the detection path's kernels, one launch per stage: real
+// row FFTs of the float level (two rows per complex FFT), column FFTs of the half spectra,
+// inverse columns with the quadrature bank G, inverse rows with the complex-cell magnitude
+// over the Hc x Wc region, then the cells stencil.
+int group_forward(bimp_diff* d, int first, int n, float* resp, hipStream_t s) {
+    const DScale& x0 = d->sc[first];
+    const auto& g = x0.g;
+    bimpk::ScaleSet ss;
+    ss.n = n;
+    bimpk::CellScale cs[bimpk::kCellMaxScales];
+    for (int k = 0; k < n; ++k) {
+        const DScale& x = d->sc[first + k];
+        ss.kX[k] = x.g.kX;
+        ss.G[k] = x.G;
+        ss.DT[k] = x.DT;
+        ss.C[k] = x.Cf;
+        cs[k] = bimpk::CellScale{x.Cf, x.Hc, x.ldc, g.FH, g.FW, g.H, g.W, x.cp,
+                                 resp + x.respOff, d->Sf + x.respOff, d->maxslot};
+    }
+    const int ncols = g.FW / 2 + 1;
+    const size_t rstride = (size_t)g.H * ncols;
+    const size_t tstride = (size_t)kNumOri * x0.Hc * g.FW;
+    DCHK(d, bimpk::launch_fwd_rows_f32(g.FW, d->lev[g.level], (size_t)d->lw[g.level], g.H, g.W, ss,
+                                       255.0f * d->opt.input_scale, d->R, ncols, rstride,
+                                       d->tw[g.FW], s));
+    DCHK(d, bimpk::launch_fwd_cols(g.FH, d->R, ncols, rstride, g.H, ss, ncols, d->tw[g.FH], s));
+    DCHK(d, bimpk::launch_inv_cols(g.FH, ss, g.FW, x0.Hc, d->T, tstride, d->tw[g.FH], s));
+    DCHK(d, bimpk::launch_inv_rows(g.FW, g.FH, ss, d->T, tstride, x0.Hc, x0.Wc, x0.ldc,
+                                   d->tw[g.FW], s));
+    DCHK(d, bimpk::launch_cells(cs, n, s));
     return BIMP_OK;
 }
 
@@ -204,11 +255,29 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         }
         x.respOff = d->respSize;
         d->respSize += (size_t)x.g.H * x.g.W;
+        // the complex-cell region the cells stencil reads (the detection context's rule)
+        x.Wc = std::min(x.g.FW, x.g.W + 1);
+        x.Hc = std::min(x.g.FH, x.g.H + (x.g.W == x.g.FW ? 2 : 1));
+        x.ldc = (x.Wc + 31) / 32 * 32;
+
         d->maxLevel = std::max(d->maxLevel, x.g.level);
         d->Pmax = std::max(d->Pmax, (size_t)x.g.FH * x.g.FW);
         sizes.push_back(x.g.FH);
         sizes.push_back(x.g.FW);
         d->sc.push_back(x);
+    }
+    // launch groups: consecutive scales (sorted by wavelength) of one level and transform size
+    for (int i = 0; i < (int)d->sc.size();) {
+        const auto& g = d->sc[i].g;
+        int n = 1;
+        while (i + n < (int)d->sc.size() && n < bimpk::kMaxBatch && n < bimpk::kCellMaxScales &&
+               d->sc[i + n].g.level == g.level && d->sc[i + n].g.FH == g.FH &&
+               d->sc[i + n].g.FW == g.FW)
+            ++n;
+        d->groups.emplace_back(i, n);
+        d->rSize = std::max(d->rSize, (size_t)n * g.H * (g.FW / 2 + 1));
+        d->tSize = std::max(d->tSize, (size_t)n * kNumOri * d->sc[i].Hc * g.FW);
+        i += n;
     }
     d->lw.assign(d->maxLevel + 1, 0);
     d->lh.assign(d->maxLevel + 1, 0);
@@ -231,6 +300,9 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         maxTaps = std::max(maxTaps, kk);
         d->maxBlocks = std::max(d->maxBlocks, bimpk::diff_param_blocks(x.g.kW));
         if (hipMalloc(&x.Q, (size_t)kNumOri * x.g.FH * x.g.FW * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&x.G, (size_t)bimpk::kUniqueOri * x.g.FH * x.g.FW * sizeof(float)) != hipSuccess ||
+            hipMalloc(&x.DT, (size_t)(x.g.FW / 2 + 1) * x.g.FH * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&x.Cf, (size_t)kNumOri * x.Hc * x.ldc * sizeof(float)) != hipSuccess ||
             hipMalloc(&x.dK, 4 * kNumOri * kk * sizeof(float)) != hipSuccess)
             return bail(BIMP_E_ALLOC);
     }
@@ -243,6 +315,9 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         hipMalloc(&d->S, 2 * P * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->taps, 2 * kNumOri * maxTaps * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->maxslot, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&d->R, d->rSize * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->T, d->tSize * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->Sf, d->respSize * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->partials, 2 * (size_t)d->maxBlocks * d->sc.size() * sizeof(float)) != hipSuccess)
         return bail(BIMP_E_ALLOC);
     std::sort(sizes.begin(), sizes.end());
@@ -270,6 +345,9 @@ int bimp_diff_destroy(bimp_diff* d) {
     (void)hipDeviceSynchronize();
     for (auto& x : d->sc) {
         (void)hipFree(x.Q);
+        (void)hipFree(x.G);
+        (void)hipFree(x.DT);
+        (void)hipFree(x.Cf);
         (void)hipFree(x.dK);
     }
     for (auto* p : d->lev) (void)hipFree(p);
@@ -277,7 +355,7 @@ int bimp_diff_destroy(bimp_diff* d) {
     for (auto& kv : d->tw) (void)hipFree(kv.second);
     for (void* p : {(void*)d->Dhat, (void*)d->acc, (void*)d->Z, (void*)d->C,
                     (void*)d->gC, (void*)d->Wm, (void*)d->S, (void*)d->taps, (void*)d->maxslot,
-                    (void*)d->partials})
+                    (void*)d->partials, (void*)d->R, (void*)d->T, (void*)d->Sf})
         (void)hipFree(p);
     delete d;
     return BIMP_OK;
@@ -299,8 +377,9 @@ int bimp_diff_forward(bimp_diff* d, const float* d_img, int nimg, float* d_resp,
     for (int b = 0; b < nimg; ++b) {
         int st = pyramid(d, d_img + b * isz, s);
         if (st) return st;
-        for (auto& x : d->sc)
-            if ((st = scale_forward(d, x, d_resp + b * d->respSize + x.respOff, s))) return st;
+        for (const auto& gr : d->groups)
+            if ((st = group_forward(d, gr.first, gr.second, d_resp + b * d->respSize, s)))
+                return st;
     }
     return BIMP_OK;
 }
@@ -331,7 +410,8 @@ int bimp_diff_backward(bimp_diff* d, const float* d_img, int nimg, const float* 
             if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, false, s))) return st;
             DCHK(d, bimpk::diff_accum_conjq(x.Q, d->Z, d->acc, P, invN, s));
             if ((st = fft2(d, d->acc, 1, g.FH, g.FW, true, s))) return st;
-            DCHK(d, bimpk::diff_pad_adj(d->acc, g.H, g.W, g.FH, g.FW, g.kX, sc, d->glev[g.level], s));
+            DCHK(d, bimpk::diff_pad_adj(d->acc, g.H, g.W, g.FH, g.FW, g.kX, sc, d->glev[g.level],
+                                        d->lw[g.level], s));
             DCHK(d, bimpk::diff_mul_conjd(d->Z, d->Dhat, P, invN, s));
             if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, true, s))) return st;
             DCHK(d, bimpk::diff_param_partials(d->Z, g.kW, g.kX, g.FH, g.FW, x.dK,

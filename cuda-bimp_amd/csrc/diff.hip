@@ -156,12 +156,14 @@ __global__ void k_pyr_down_adj(const float* __restrict__ gdst, int dw, int dh,
 
 __device__ __forceinline__ int clamp_src(int y, int H, int k) { return y < H ? y : (y < H + k ? H - 1 : 0); }
 
-// Pd = s * pad(L) as complex (imaginary part 0).
-__global__ void k_pad_fwd(const float* __restrict__ L, int H, int W, int FH, int FW, int k,
-                          float s, float2* __restrict__ out) {
+// Pd = s * pad(L) as complex (imaginary part 0).  L is the top-left H x W region of a level
+// image with `pitch` floats per row (a level is (n+1)/2 wide, its data n/2).
+__global__ void k_pad_fwd(const float* __restrict__ L, int pitch, int H, int W, int FH, int FW,
+                          int k, float s, float2* __restrict__ out) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= FW) return;
-    out[(size_t)y * FW + x] = make_float2(s * L[(size_t)clamp_src(y, H, k) * W + clamp_src(x, W, k)], 0.f);
+    out[(size_t)y * FW + x] =
+            make_float2(s * L[(size_t)clamp_src(y, H, k) * pitch + clamp_src(x, W, k)], 0.f);
 }
 
 // Source rows of the padded image that read data row r: r itself, the band [H, H+k) for
@@ -178,9 +180,9 @@ __device__ __forceinline__ void pad_band(int r, int H, int F, int k, int* lo1, i
     }
 }
 
-// gL += s * pad^T(Re gPd)
+// gL += s * pad^T(Re gPd) over the top-left H x W region of gL (`pitch` floats per row)
 __global__ void k_pad_adj(const float2* __restrict__ g, int H, int W, int FH, int FW, int k,
-                          float s, float* __restrict__ gL) {
+                          float s, float* __restrict__ gL, int pitch) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W) return;
     int ylo, yhi, xlo, xhi;
@@ -193,7 +195,28 @@ __global__ void k_pad_adj(const float2* __restrict__ g, int H, int W, int FH, in
         acc += row[x].x;
         for (int xx = xlo; xx < xhi; ++xx) acc += row[xx].x;
     }
-    gL[(size_t)y * W + x] += s * acc;
+    gL[(size_t)y * pitch + x] += s * acc;
+}
+
+// G_o(ky, kx) = Re Q_o(ky, kx) / (FH*FW) for o = 0..4, stored kx-major (G[o][kx][ky]): the
+// quadrature bank layout of the detection path's inverse column pass, from the spectra Q of
+// the current parameters.  32 x 32 tiles transposed through LDS.
+__global__ void __launch_bounds__(256) k_q_to_g(const float2* __restrict__ Q, int FH, int FW,
+                                               float invN, float* __restrict__ G) {
+    __shared__ float tile[32][33];
+    const int o = blockIdx.z, kx0 = blockIdx.x * 32, ky0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    const float2* q = Q + (size_t)o * FH * FW;
+    for (int r = ty; r < 32; r += 8) {
+        const int ky = ky0 + r, kx = kx0 + tx;
+        if (ky < FH && kx < FW) tile[r][tx] = q[(size_t)ky * FW + kx].x * invN;
+    }
+    __syncthreads();
+    float* g = G + (size_t)o * FW * FH;
+    for (int r = ty; r < 32; r += 8) {
+        const int kx = kx0 + r, ky = ky0 + tx;
+        if (ky < FH && kx < FW) g[(size_t)kx * FH + ky] = tile[tx][r];
+    }
 }
 
 // Padded complex kernels of the 8 orientations: tap (r, c) sits at ((r-kX) mod FH,
@@ -471,15 +494,20 @@ hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int
     return hipGetLastError();
 }
 
-hipError_t diff_pad(const float* L, int H, int W, int FH, int FW, int k, float sc, float2* out,
-                    hipStream_t s) {
-    k_pad_fwd<<<dim3((FW + 127) / 128, FH), 128, 0, s>>>(L, H, W, FH, FW, k, sc, out);
+hipError_t diff_pad(const float* L, int pitch, int H, int W, int FH, int FW, int k, float sc,
+                    float2* out, hipStream_t s) {
+    k_pad_fwd<<<dim3((FW + 127) / 128, FH), 128, 0, s>>>(L, pitch, H, W, FH, FW, k, sc, out);
     return hipGetLastError();
 }
 
 hipError_t diff_pad_adj(const float2* g, int H, int W, int FH, int FW, int k, float sc,
-                        float* gL, hipStream_t s) {
-    k_pad_adj<<<dim3((W + 127) / 128, H), 128, 0, s>>>(g, H, W, FH, FW, k, sc, gL);
+                        float* gL, int pitch, hipStream_t s) {
+    k_pad_adj<<<dim3((W + 127) / 128, H), 128, 0, s>>>(g, H, W, FH, FW, k, sc, gL, pitch);
+    return hipGetLastError();
+}
+
+hipError_t diff_q_to_g(const float2* Q, int FH, int FW, float invN, float* G, hipStream_t s) {
+    k_q_to_g<<<dim3((FW + 31) / 32, (FH + 31) / 32, kUniqueOri), 256, 0, s>>>(Q, FH, FW, invN, G);
     return hipGetLastError();
 }
 

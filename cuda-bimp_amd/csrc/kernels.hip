@@ -2,7 +2,7 @@
 //
 // Stage map (reference: DoFullGPUFiltering, bimp/src/cuda_bimp.cu:462-559):
 //   k_pyr_down     pyramid level (cv::cuda::pyrDown, cuda_bimp.cu:473-479), once per level
-//   k_fwd_rows     u8 -> float, clamp padding in x, real row FFTs (two rows per complex FFT)
+//   k_fwd_rows     u8 (or float) -> float, clamp padding in x, real row FFTs (two rows per complex FFT)
 //                  (convertTo + collectCVMemory + padDataClampToBorder + cufftExecR2C, :480-499)
 //   k_fwd_cols     clamp padding in y, column FFTs -> image spectrum D, stored column-major
 //   k_inv_cols     D * G_o (quadrature bank, 1/(FH*FW) folded in), inverse column FFTs
@@ -68,9 +68,10 @@ struct RowCfg {
 // Two image rows (a, b) form one complex vector a + i*b; after the FFT the half spectra of
 // both are separated with Z(k) and conj(Z(N-k)).  Columns beyond W repeat column W-1 for kX
 // samples, then column 0 (padDataClampToBorder, convolutionFFT2D.cuh:56-85).  blockIdx.y
-// selects the scale of the batch (scales of one pyramid level differ only in kX).
-template <int FW>
-__global__ void __launch_bounds__(64) k_fwd_rows(const uint8_t* __restrict__ img, size_t pitch,
+// selects the scale of the batch (scales of one pyramid level differ only in kX).  Pixels are
+// uint8 (detection) or float (the differentiation form's images); pitch counts pixels.
+template <int FW, class Pix>
+__global__ void __launch_bounds__(64) k_fwd_rows(const Pix* __restrict__ img, size_t pitch,
                                                  int H, int W, ScaleSet ss, float scale,
                                                  float2* __restrict__ Rbase, int ldr,
                                                  size_t rstride, const float2* __restrict__ tw) {
@@ -83,8 +84,8 @@ __global__ void __launch_bounds__(64) k_fwd_rows(const uint8_t* __restrict__ img
     const int ya = 2 * (blockIdx.x * RowCfg<FW>::VPB + g), yb = ya + 1;
     const bool haveA = ya < H, haveB = yb < H;
     float* my = lds + (size_t)g * 2 * FW;
-    const uint8_t* rowA = img + (size_t)(haveA ? ya : 0) * pitch;
-    const uint8_t* rowB = img + (size_t)(haveB ? yb : 0) * pitch;
+    const Pix* rowA = img + (size_t)(haveA ? ya : 0) * pitch;
+    const Pix* rowB = img + (size_t)(haveB ? yb : 0) * pitch;
     // unconditional loads (rows clamped above) so that all of them are in flight at once
     const float sa = haveA ? scale : 0.f, sb = haveB ? scale : 0.f;
     float2 v[C::E];
@@ -1205,7 +1206,10 @@ static hipError_t prepare_n() {
     hipError_t e;
     const int rowLds = RowCfg<N>::VPB * N * (int)sizeof(float2);  // fwd rows stage complex output
     const int colLds = ColCfg<N>::CB * ColCfg<N>::S * (int)sizeof(float2);
-    if ((e = hipFuncSetAttribute((const void*)k_fwd_rows<N>,
+    if ((e = hipFuncSetAttribute((const void*)k_fwd_rows<N, uint8_t>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, rowLds)) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)k_fwd_rows<N, float>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, rowLds)) != hipSuccess)
         return e;
     if ((e = hipFuncSetAttribute((const void*)k_inv_rows<N>,
@@ -1252,21 +1256,34 @@ hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, ui
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W,
-                           const ScaleSet& ss, float scale, float2* R, int ldr, size_t rstride,
-                           const float2* tw, hipStream_t s) {
+template <class Pix>
+static hipError_t fwd_rows(int FW, const Pix* img, size_t pitch, int H, int W, const ScaleSet& ss,
+                           float scale, float2* R, int ldr, size_t rstride, const float2* tw,
+                           hipStream_t s) {
 #define X(N)                                                                                 \
     if (FW == N) {                                                                           \
         const int vpb = RowCfg<N>::VPB;                                                      \
         const int pairs = (H + 1) / 2;                                                       \
         dim3 grid((pairs + vpb - 1) / vpb, ss.n);                                            \
-        k_fwd_rows<N><<<grid, RowCfg<N>::THREADS, vpb * N * sizeof(float2), s>>>(            \
+        k_fwd_rows<N, Pix><<<grid, RowCfg<N>::THREADS, vpb * N * sizeof(float2), s>>>(       \
             img, pitch, H, W, ss, scale, R, ldr, rstride, tw);                               \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W,
+                           const ScaleSet& ss, float scale, float2* R, int ldr, size_t rstride,
+                           const float2* tw, hipStream_t s) {
+    return fwd_rows(FW, img, pitch, H, W, ss, scale, R, ldr, rstride, tw, s);
+}
+
+hipError_t launch_fwd_rows_f32(int FW, const float* img, size_t pitch, int H, int W,
+                               const ScaleSet& ss, float scale, float2* R, int ldr,
+                               size_t rstride, const float2* tw, hipStream_t s) {
+    return fwd_rows(FW, img, pitch, H, W, ss, scale, R, ldr, rstride, tw, s);
 }
 
 hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, size_t rstride, int H,

@@ -111,6 +111,10 @@ hipError_t launch_pyr_down(const uint8_t* src, int sw, int sh, size_t spitch, ui
 hipError_t launch_fwd_rows(int FW, const uint8_t* img, size_t pitch, int H, int W,
                            const ScaleSet& ss, float scale, float2* R, int ldr, size_t rstride,
                            const float2* tw, hipStream_t s);
+// The same pass over a float image (pitch in floats), e.g. a differentiation-form level.
+hipError_t launch_fwd_rows_f32(int FW, const float* img, size_t pitch, int H, int W,
+                               const ScaleSet& ss, float scale, float2* R, int ldr,
+                               size_t rstride, const float2* tw, hipStream_t s);
 hipError_t launch_fwd_cols(int FH, const float2* R, int ldr, size_t rstride, int H,
                            const ScaleSet& ss, int ncols, const float2* tw, hipStream_t s);
 hipError_t launch_inv_cols(int FH, const ScaleSet& ss, int FW, int Hc, float2* T,

@@ -202,7 +202,7 @@ class DiffRef:
             gPd = np.real(np.fft.ifft2(acc)) * self.sc
             gL = np.zeros((s.dataH, s.dataW))
             np.add.at(gL, (f["cy"][:, None], f["cx"][None, :]), gPd)
-            glev[s.level] += gL
+            glev[s.level][: s.dataH, : s.dataW] += gL  # the data is the level's top-left crop
         for l in range(self.maxlevel, 0, -1):
             Ay, Ax = mats[l - 1]
             glev[l - 1] += Ay.T @ glev[l] @ Ax

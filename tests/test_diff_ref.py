@@ -25,8 +25,10 @@ def test_forward_matches_the_uint8_oracle_at_level_0():
         assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
 
 
-def test_image_gradient_matches_central_differences():
-    ref, img, targets = _case()
+@pytest.mark.parametrize("w,h", [(48, 40), (49, 41)])
+def test_image_gradient_matches_central_differences(w, h):
+    """(49, 41): the level-1 data (24 x 20) is the top-left crop of a 25 x 21 level."""
+    ref, img, targets = _case(w, h)
     loss, g, _ = ref.gradient(img, targets)
     rng = np.random.default_rng(0)
     for _ in range(3):
