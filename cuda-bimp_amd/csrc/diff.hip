@@ -36,15 +36,19 @@ __global__ void __launch_bounds__(kRowThreads) k_c2c_rows(float2* __restrict__ d
     constexpr int GPB = kRowThreads / C::TPV;
     extern __shared__ float lds[];
     const int grp = threadIdx.x / C::TPV, lane = threadIdx.x % C::TPV;
+    // every group transforms (a spare one row 0, unstored): a 128-lane group's stages take
+    // the workgroup barrier
     const int row = blockIdx.x * GPB + grp;
-    if (row >= rows) return;  // whole groups only: a group never spans two wavefronts
-    float2* p = data + (size_t)row * N;
+    const bool act = row < rows;
+    float2* p = data + (size_t)(act ? row : 0) * N;
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = p[in_index<N>(e, lane)];
     bimpdev::fft<N, INV>(v, lds + grp * bimpdev::fft_lds_floats<N>(), lane, tw);
+    if (act) {
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) p[out_index<N>(e, lane)] = v[e];
+        for (int e = 0; e < C::E; ++e) p[out_index<N>(e, lane)] = v[e];
+    }
 }
 
 // In-place complex transforms of the columns of `nb` N x W maps ([b][r][c], r < N): a
@@ -78,7 +82,7 @@ __global__ void __launch_bounds__(CColCfg<N>::THREADS) k_c2c_cols(float2* __rest
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = my[in_index<N>(e, lane)];
-    bimpdev::stage_sync();
+    bimpdev::stage_sync<C::TPV>();
     bimpdev::fft<N, INV>(v, reinterpret_cast<float*>(my), lane, tw);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) my[out_index<N>(e, lane)] = v[e];

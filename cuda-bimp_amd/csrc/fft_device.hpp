@@ -1,7 +1,8 @@
 // fft_device.hpp -- register-resident Stockham FFT for gfx950 wavefronts.
 //
-// A length-N transform is owned by a group of TPV lanes (TPV = 64 for N >= 512: one
-// wavefront per vector); every lane keeps E = N/TPV complex values in registers.  Each
+// A length-N transform is owned by a group of TPV lanes (TPV = 64 for 512 <= N <= 3584: one
+// wavefront per vector; TPV = 128 for 4096: two wavefronts, so that a lane keeps 32 values and
+// not 64); every lane keeps E = N/TPV complex values in registers.  Each
 // radix-R stage does E/R butterflies per lane entirely in registers; between stages the
 // group exchanges data through its own LDS region (N floats: real parts, then imaginary
 // parts), so the only cross-lane traffic is one LDS write + read per stage boundary.
@@ -11,8 +12,9 @@
 //   stage output of butterfly j, leg r              : (j/Ns)*Ns*R + j%Ns + r*Ns
 // so the first stage reads element in_index(e) and the last stage produces element
 // out_index(e) in register slot e -- both are unit-stride across lanes, which makes global
-// loads and stores of a vector coalesced without any extra shuffle.  A group never spans
-// two wavefronts, so stage boundaries need only wavefront-level synchronisation.
+// loads and stores of a vector coalesced without any extra shuffle.  A group of up to 64
+// lanes lies in one wavefront, so its stage boundaries need only wavefront-level
+// synchronisation; a 128-lane group synchronises with the workgroup barrier (stage_sync).
 //
 // Twiddles come from a per-N packed table (TwLayout) computed in FP64 on the host and
 // rounded to FP32; the inverse transform uses their conjugates (unnormalised).
@@ -253,7 +255,7 @@ BIMP_FFT_CFG(2048, 64, 16, 16, 8)
 BIMP_FFT_CFG(2560, 64, 40, 8, 8)
 BIMP_FFT_CFG(3072, 64, 48, 8, 8)
 BIMP_FFT_CFG(3584, 64, 56, 8, 8)
-BIMP_FFT_CFG(4096, 64, 16, 16, 16)
+BIMP_FFT_CFG(4096, 128, 16, 16, 16)
 #undef BIMP_FFT_CFG
 
 template <class L>
@@ -300,13 +302,20 @@ __device__ __forceinline__ int out_index(int e, int lane) {
     return lane + C::TPV * (e / R) + (e % R) * (N / R);
 }
 
-// LDS ordering between the stages of one group.  A group never spans two wavefronts (TPV
-// divides 64) and owns its LDS region, so a wavefront-scope release/acquire around a wave
-// barrier orders the exchange (the same sequence HIP uses for __syncwarp).
+// LDS ordering between the stages of one group of TPV lanes.  A group of up to 64 lanes lies
+// in one wavefront (TPV divides 64) and owns its LDS region, so a wavefront-scope
+// release/acquire around a wave barrier orders the exchange (the same sequence HIP uses for
+// __syncwarp).  A 128-lane group spans two wavefronts and takes the workgroup barrier: every
+// wavefront of a workgroup that runs such transforms runs the same number of them.
+template <int TPV>
 __device__ __forceinline__ void stage_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (TPV > 64) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 // Twiddles copied into LDS by a workgroup that runs many transforms (tw_to_lds below): each
@@ -404,7 +413,7 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
             for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R, VMAX>(base + r * Ns)] = v[m * R + r].x;
         }
     }
-    stage_sync();
+    stage_sync<C::TPV>();
     float im[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) im[e] = v[e].y;
@@ -414,7 +423,7 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[m * R2 + r].x = lds[lds_pad<Ns, R, VMAX>(j + r * (N / R2))];
     }
-    stage_sync();
+    stage_sync<C::TPV>();
     if constexpr (kVecFirst<Ns, R, VMAX>) {
 #pragma unroll
         for (int m = 0; m < C::E / R; ++m) {
@@ -433,14 +442,14 @@ __device__ __forceinline__ void exchange(float2 (&v)[FFTCfg<N>::E], float* lds, 
             for (int r = 0; r < R; ++r) lds[lds_pad<Ns, R, VMAX>(base + r * Ns)] = im[m * R + r];
         }
     }
-    stage_sync();
+    stage_sync<C::TPV>();
 #pragma unroll
     for (int m = 0; m < C::E / R2; ++m) {
         const int j = lane + C::TPV * m;
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[m * R2 + r].y = lds[lds_pad<Ns, R, VMAX>(j + r * (N / R2))];
     }
-    stage_sync();
+    stage_sync<C::TPV>();
 }
 
 template <int N, bool INV, int Ns, class L, int VMAX>

@@ -61,7 +61,7 @@ __global__ void k_pyr_down(const uint8_t* __restrict__ src, int sw, int sh, size
 template <int N>
 struct RowCfg {
     static constexpr int TPV = FFTCfg<N>::TPV;
-    static constexpr int THREADS = 64;  // one wavefront per workgroup
+    static constexpr int THREADS = TPV > 64 ? TPV : 64;  // one wavefront, or one 128-lane group
     static constexpr int VPB = THREADS / TPV;
 };
 
@@ -71,7 +71,7 @@ struct RowCfg {
 // selects the scale of the batch (scales of one pyramid level differ only in kX).  Pixels are
 // uint8 (detection) or float (the differentiation form's images); pitch counts pixels.
 template <int FW, class Pix>
-__global__ void __launch_bounds__(64) k_fwd_rows(const Pix* __restrict__ img, size_t pitch,
+__global__ void __launch_bounds__(RowCfg<FW>::THREADS) k_fwd_rows(const Pix* __restrict__ img, size_t pitch,
                                                  int H, int W, ScaleSet ss, float scale,
                                                  float2* __restrict__ Rbase, int ldr,
                                                  size_t rstride, const float2* __restrict__ tw) {
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(64) k_fwd_rows(const Pix* __restrict__ img, si
     float2* z = reinterpret_cast<float2*>(my);
 #pragma unroll
     for (int e = 0; e < C::E; ++e) z[out_index<FW>(e, lane)] = v[e];
-    stage_sync();
+    stage_sync<C::TPV>();
     for (int k = lane; k <= FW / 2; k += C::TPV) {
         const float2 z1 = z[k];
         const float2 z2 = z[k == 0 ? 0 : FW - k];
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(ColCfg<FH>::THREADS) k_fwd_cols(
             if (CC::HALVES == 1 || ((e % R1) >= R1 / 2) == (h == 1))
                 v[e] = my[in_index<FH>(e, lane) - h * HN];
     }
-    stage_sync();
+    stage_sync<C::TPV>();
     fft<FH, false>(v, reinterpret_cast<float*>(my), lane, tw);
     const int kx = kx0 + g;
     if (kx < ncols) {
@@ -396,7 +396,7 @@ __device__ __forceinline__ void load_t_row(float2 (&v)[FFTCfg<FW>::E], const flo
 }
 
 template <int FW>
-__global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __restrict__ Tbase,
+__global__ void __launch_bounds__(RowCfg<FW>::THREADS) k_inv_rows(ScaleSet ss, const float2* __restrict__ Tbase,
                                                  size_t tstride, int Hc, int Wc, int ldc, int tsh,
                                                  const float2* __restrict__ tw) {
     using C = FFTCfg<FW>;
@@ -420,17 +420,18 @@ __global__ void __launch_bounds__(64) k_inv_rows(ScaleSet ss, const float2* __re
     }
 }
 
-// Persistent form for transforms of one full wavefront (TPV = 64) up to 2048 points: each
-// workgroup copies the twiddle table into LDS once and its WPB wavefronts then walk units of
-// WPB consecutive rows (one row per wavefront) of one orientation of one scale, stride
-// gridDim.x.  Same arithmetic as k_inv_rows.
+// Persistent form for transforms of one or two full wavefronts (TPV = 64 or 128): each
+// workgroup copies the twiddle table into LDS once and its RPU groups then walk units of RPU
+// consecutive rows (one row per group) of one orientation of one scale, stride gridDim.x.
+// Same arithmetic as k_inv_rows.
 template <int FW>
 struct InvRowCfg {
     static constexpr int WPB = 4;
     static constexpr int THREADS = 64 * WPB;
-    static constexpr size_t MAIN = (size_t)WPB * fft_lds_floats<FW>() * sizeof(float);
+    static constexpr int RPU = THREADS / FFTCfg<FW>::TPV;  // rows per unit
+    static constexpr size_t MAIN = (size_t)RPU * fft_lds_floats<FW>() * sizeof(float);
     static constexpr size_t LDS = MAIN + tw_lds_bytes<FW>();
-    static constexpr bool PERSISTENT = FFTCfg<FW>::TPV == 64;
+    static constexpr bool PERSISTENT = FFTCfg<FW>::TPV >= 64;
 };
 
 template <int FW>
@@ -439,21 +440,21 @@ __global__ void __launch_bounds__(InvRowCfg<FW>::THREADS) k_inv_rows_p(
         int tsh, int units, const float2* __restrict__ tw) {
     using C = FFTCfg<FW>;
     using RC = InvRowCfg<FW>;
-    static_assert(C::TPV == 64, "one row per wavefront");
+    static_assert(C::TPV >= 64 && RC::THREADS % C::TPV == 0, "whole groups of wavefronts");
     extern __shared__ float lds[];
-    const int w = threadIdx.x / 64, lane0 = threadIdx.x % 64;
+    const int w = threadIdx.x / C::TPV, lane0 = threadIdx.x % C::TPV;
     float* my = lds + (size_t)w * fft_lds_floats<FW>();
     float4* twl = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds) + RC::MAIN);
     tw_to_lds<FW>(tw, twl, threadIdx.x, RC::THREADS);
     __syncthreads();
-    const int nrb = (Hc + RC::WPB - 1) / RC::WPB;
+    const int nrb = (Hc + RC::RPU - 1) / RC::RPU;
 #pragma unroll 1
     for (int u = blockIdx.x; u < units; u += gridDim.x) {
         const int rb = u % nrb, o = (u / nrb) % kNumOri, sidx = u / (nrb * kNumOri);
         // lane-derived indices are recomputed per unit rather than kept live across units
         int lane = lane0;
         asm volatile("" : "+v"(lane));
-        const int y = rb * RC::WPB + w;
+        const int y = rb * RC::RPU + w;
         const bool active = y < Hc;
         float2 v[C::E];
         load_t_row<FW>(v, Tbase + (size_t)sidx * tstride + (size_t)o * Hc * FW, Hc,
@@ -1332,7 +1333,7 @@ hipError_t launch_inv_rows(int FW, int FH, const ScaleSet& ss, const float2* T, 
     if (FW == N) {                                                                           \
         if constexpr (InvRowCfg<N>::PERSISTENT) {                                            \
             using RC = InvRowCfg<N>;                                                         \
-            const int units = (Hc + RC::WPB - 1) / RC::WPB * kNumOri * ss.n;                 \
+            const int units = (Hc + RC::RPU - 1) / RC::RPU * kNumOri * ss.n;                 \
             const int grid = std::min(units, inv_rows_slots<N>());                           \
             if (grid <= 0) return hipErrorInvalidValue;                                      \
             k_inv_rows_p<N><<<grid, RC::THREADS, RC::LDS, s>>>(ss, T, tstride, Hc, Wc, ldc,  \

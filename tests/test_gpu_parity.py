@@ -201,9 +201,11 @@ def test_repeat_frames_identical(B):
 
 
 @pytest.mark.parametrize("w,h,fw,fh", [(2600, 200, 3072, 256), (3100, 96, 3584, 128),
-                                       (48, 2600, 64, 3072), (40, 3100, 64, 3584)])
+                                       (48, 2600, 64, 3072), (40, 3100, 64, 3584),
+                                       (3700, 72, 4096, 128), (56, 3700, 64, 4096)])
 def test_transform_sizes_3072_3584(B, w, h, fw, fh):
-    """Sides that snap to 3072 (48*8*8, 48 = 16*3) and 3584 (56*8*8, 56 = 8*7) in both directions."""
+    """Sides that snap to 3072 (48*8*8, 48 = 16*3), 3584 (56*8*8, 56 = 8*7) and 4096 (two
+    wavefronts per transform) in both directions."""
     lam = O.BASE_BANK[:2]
     cs, orc = _pair(B, w, h, lam)
     assert (cs.ks[0].fft_w, cs.ks[0].fft_h) == (fw, fh)
