@@ -5,7 +5,8 @@ lambda = 4*2^(k/4) x 8 orientations, plain FFT-FP32 path.  A step is one full fr
 filter bank, complex cells, end-stopping, inhibition, NMS and keypoint compaction, from a
 device-resident frame to a device-resident keypoint list and count.
 
-Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel, CPU baseline).
+Prints one JSON line (frames/s, per-kernel roofline of the dominant kernel with its HBM
+traffic measured by two rocprofv3 counter passes over a child process, CPU baseline).
 The same line carries the precision form (BF16 / FP16 direct convolution on the matrix cores
 for every scale): its frames/s, speed-up over the plain
 path, MFMA roofline, per-scale map error and keypoint agreement against the FP64 oracle,
@@ -388,6 +389,88 @@ def load_traffic(kernel):
         return None
 
 
+COPY_MIB = 256
+
+
+def traffic_probe(form):
+    """The workload of a `--traffic-probe` child: five frames of the form on one stream, then
+    k_copy moving COPY_MIB MiB (the counters' calibration), nothing printed."""
+    import bimp_amd as B
+    img = B.synth_frame(W, H, NSHAPES, SEED)
+    opt = B.default_options()
+    if form != "plain":
+        opt.precision = B.PRECISION_BF16 if form.startswith("bf16") else B.PRECISION_FP16
+        opt.direct_sigmas = 8.0 if form.endswith("_8sigma") else 0.0
+    opt.flags = B.FLAG_SINGLE_STREAM
+    cs = B.CudaStuff(W, H, bank(), opt)
+    st = B.Stream()
+    d_img = B.DeviceBuffer(W * H)
+    d_img.upload(img)
+    cap = B.KP_MAX_POINTS
+    d_out = B.DeviceBuffer(cap * B.KEYPOINT_DTYPE.itemsize)
+    cs.time_detect(d_img, W, d_out, cap, 5, st)
+    nb = COPY_MIB << 20
+    a, b = B.DeviceBuffer(nb), B.DeviceBuffer(nb)
+    B.lib.bimp_copy_kernel(b.ptr, a.ptr, nb, st.handle)
+    st.sync()
+
+
+def measure_traffic(kernel, form, timeout=180):
+    """HBM bytes per launch of `kernel`, measured now: two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE: KiB per dispatch) over a `--traffic-probe` child, each counter scaled by
+    what it reads for k_copy moving a known COPY_MIB MiB in the same child.  Returns
+    (bytes, note); bytes is None when a pass fails."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    per = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="bimp_traffic_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", "--form", form]
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                 cwd="/tmp", env=env, start_new_session=True)
+            try:
+                p.wait(timeout=timeout)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.wait()
+                return None, f"{ctr} pass exceeded {timeout} s"
+            # the profiler may end abnormally after its results are written: read what is there
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                return None, f"{ctr} pass wrote no counters (exit {p.returncode})"
+            tot, ids = {"k": 0.0, "copy": 0.0}, {"k": set(), "copy": set()}
+            with open(files[0]) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != ctr:
+                        continue
+                    name = r.get("Kernel_Name", "")
+                    key = "k" if kernel in name else ("copy" if "k_copy" in name else None)
+                    if key:
+                        tot[key] += float(r["Counter_Value"])
+                        ids[key].add(r["Dispatch_Id"])
+            if not ids["k"] or not ids["copy"]:
+                return None, f"{ctr} pass: no dispatch of {kernel} or k_copy"
+            per[ctr] = (tot["k"] / len(ids["k"]), tot["copy"] / len(ids["copy"]))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    kib = COPY_MIB * 1024.0
+    fcal, wcal = per["FETCH_SIZE"][1] / kib, per["WRITE_SIZE"][1] / kib
+    byts = (per["FETCH_SIZE"][0] / fcal + per["WRITE_SIZE"][0] / wcal) * 1024.0
+    return int(byts), (f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over five frames in a "
+                       f"child of this run; k_copy of {COPY_MIB} MiB reads FETCH_SIZE x{fcal:.3f} "
+                       f"and WRITE_SIZE x{wcal:.3f} of its bytes")
+
+
 def batch_form_child(timeout=600):
     """Runs the batching form in a child process (`bench.py --only-batch`), so that a failure
     there is reported in its entry instead of ending the benchmark."""
@@ -424,6 +507,10 @@ def main():
                     help="run the main line's frames on one stream (no overlap of the pyramid "
                          "levels' transforms); profiler runs use it so that kernel durations "
                          "match the per-kernel profile")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="take roofline.traffic from profiles/traffic.json instead of measuring "
+                         "it with two rocprofv3 passes (for runs that are themselves profiled)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, launch the library's plain copy kernel once over MIB "
                          "MiB (the reference point for HBM byte counters)")
@@ -441,6 +528,9 @@ def main():
         sys.exit(1)
     if args.only_batch:
         print(json.dumps(batch_form(B, B.Stream())), flush=True)
+        return
+    if args.traffic_probe:
+        traffic_probe(args.form)
         return
     img = B.synth_frame(W, H, NSHAPES, SEED)
     opt = B.default_options()
@@ -486,6 +576,14 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     frame_kernel_ms = sum(v["total_ms"] for v in kern.values()) / args.profile_frames
     balg = b_alg(cs)
+    if args.no_traffic:
+        traffic, traffic_note = load_traffic(dom), "profiles/traffic.json (committed summary)"
+    else:
+        traffic, traffic_note = measure_traffic(dom, args.form)
+        if traffic is None:
+            traffic_note = "not measured: " + traffic_note
+        print(f"traffic of {dom}: {traffic} bytes per launch ({traffic_note})", file=sys.stderr,
+              flush=True)
 
     out = {
         "metric": METRIC,
@@ -509,7 +607,8 @@ def main():
                    "keypoints": int(count)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(dom),
+                     "traffic": traffic,
+                     "traffic_source": traffic_note,
                      "avg_launch_us": round(avg_ms * 1e3, 2),
                      "alg_bytes_per_launch": int(bytes_per_launch),
                      "alg_bytes_model": "stage-boundary model per scale (W*H + sum 84A + 352C "
