@@ -153,3 +153,30 @@ def test_odd_size_upper_level_crop(T):
     print(f"image grad rel L2 {e_img:.2e}; params rel {e_p}", flush=True)
     assert e_img <= 1e-3
     assert np.all(e_p <= 1e-3)
+
+
+def test_4096_wide_transforms(T):
+    """3700 x 40 with one scale: 4096-point row transforms (two wavefronts per transform) in the
+    forward, the recomputation and the adjoint, against the FP64 restatement."""
+    from bimp_amd.autograd import BimpResponses
+    w, h, lams = 3700, 40, (4.0,)
+    Ro = BimpResponses(w, h, lams)
+    ref = DiffRef(w, h, lams)
+    assert ref.scales[0].fftW == 4096
+    img = O.synth_frame(w, h, 30, 12).astype(np.float32) / 255.0
+    want = ref.forward(img.astype(np.float64))
+    p = T.tensor([0.56, 0.5], dtype=T.float64)
+    got = Ro(T.tensor(img[None], device="cuda"), p).cpu().numpy()[0]
+    e = np.abs(got - want[0].ravel()).max() / np.abs(want[0]).max()
+    print(f"response rel err {e:.2e}", flush=True)
+    assert e <= 1e-4
+    targets = ref.forward(img.astype(np.float64), gamma=0.6)
+    tgt = T.tensor(np.concatenate([t.ravel() for t in targets])[None].astype(np.float32),
+                   device="cuda")
+    _, gx, gp = _loss_and_grads(T, Ro, img[None], tgt, [0.56, 0.5])
+    _, rg, rth = ref.gradient(img.astype(np.float64), targets)
+    e_img = np.linalg.norm(gx[0] - rg) / np.linalg.norm(rg)
+    e_p = np.abs(gp - rth) / np.abs(rth)
+    print(f"image grad rel L2 {e_img:.2e}; params rel {e_p}", flush=True)
+    assert e_img <= 1e-3
+    assert np.all(e_p <= 1e-3)
