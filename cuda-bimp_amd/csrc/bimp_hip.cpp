@@ -421,13 +421,10 @@ int run_keypoints(bimp_ctx* c, void* out, int cap, int matrix_ld, hipStream_t s)
                                           c->blockMasks, s));
     }
     {
-        LaunchScope ls(c, s, F_NMS_SCAN, 8.0 * c->totalBlocks);
-        HIPCHK(c, bimpk::launch_nms_scan(c->blockCounts, c->totalBlocks, c->counter, s));
-    }
-    {
         LaunchScope ls(c, s, F_NMS_WRITE, 4.0 * bimpk::kNmsThreads * c->totalBlocks);
         HIPCHK(c, bimpk::launch_nms_write(c->nmsTable, (int)c->sc.size(), c->totalBlocks,
-                                          c->blockCounts, c->blockMasks, out, cap, matrix_ld, s));
+                                          c->blockCounts, c->counter, c->blockMasks, out, cap,
+                                          matrix_ld, s));
     }
     return BIMP_OK;
 }

@@ -16,12 +16,12 @@ namespace bimpi {
 
 enum Family {
     F_INIT, F_COPY_IN, F_PYR, F_FWD_ROWS, F_FWD_COLS, F_INV_COLS, F_INV_ROWS, F_CELLS,
-    F_MAX, F_NMS_COUNT, F_NMS_SCAN, F_NMS_WRITE, F_DIRECT, F_COUNT
+    F_MAX, F_NMS_COUNT, F_NMS_WRITE, F_DIRECT, F_COUNT
 };
 // Profile names follow the kernel symbols (size-templated kernels carry their length).
 inline const char* const kFamilyNames[F_COUNT] = {"k_init",      "copy_in",    "k_pyr_down",  "k_fwd_rows",
                                      "k_fwd_cols",  "k_inv_cols", "k_inv_rows",  "k_cells",
-                                     "k_max",       "k_nms_count", "k_nms_scan", "k_nms_write",
+                                     "k_max",       "k_nms_count", "k_nms_write",
                                      "k_direct"};
 
 struct Scale {

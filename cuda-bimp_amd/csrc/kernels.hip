@@ -937,51 +937,37 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_count(const NmsScale* __res
     }
 }
 
-// Exclusive scan of the per-block counts in place (one workgroup); the total goes to counter
-// and to counts[n].
-__global__ void __launch_bounds__(1024) k_nms_scan(int* __restrict__ counts, int n,
-                                                   int* __restrict__ counter) {
-    __shared__ int part[1024];
-    const int tid = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b0 = tid * per, b1 = min(n, b0 + per);
-    int sum = 0;
-    for (int i = b0; i < b1; ++i) sum += counts[i];
-    part[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    int run = part[tid] - sum;  // exclusive prefix of this thread's range
-    for (int i = b0; i < b1; ++i) {
-        const int c = counts[i];
-        counts[i] = run;
-        run += c;
-    }
-    if (tid == 1023) {
-        *counter = part[1023];
-        counts[n] = part[1023];
-    }
-}
-
-// Writes each keypoint of k_nms_count's hit masks at its row-major rank.  matrix_ld > 0
-// selects the 7-row SURF_CUDA layout (cuda_bimp.cuh:132-140) instead of keypoint structs.
+// Writes each keypoint of k_nms_count's hit masks at its row-major rank.  A block's first
+// rank is the sum of the counts of the blocks before it, summed here (a few thousand
+// cache-resident ints per block with keypoints) rather than by a scan launch; the last block
+// also stores the frame's total in counter.  matrix_ld > 0 selects the 7-row SURF_CUDA layout
+// (cuda_bimp.cuh:132-140) instead of keypoint structs.
 __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
-        const NmsScale* __restrict__ sc, int nscales, const int* __restrict__ offsets,
-        const unsigned* __restrict__ masks, void* __restrict__ out, int cap, int matrix_ld) {
+        const NmsScale* __restrict__ sc, int nscales, const int* __restrict__ counts,
+        int total_blocks, int* __restrict__ counter, const unsigned* __restrict__ masks,
+        void* __restrict__ out, int cap, int matrix_ld) {
 #pragma clang fp contract(off)
     const int blk = blockIdx.x;
+    const bool last = blk == total_blocks - 1;
+    const int mine = counts[blk];
+    if (mine == 0 && !last) return;  // uniform: no keypoints, no total to store
+    constexpr int NW = kNmsThreads / 64;
+    __shared__ int ws[NW];
+    int pre = 0;
+    for (int i = (int)threadIdx.x; i < blk; i += kNmsThreads) pre += counts[i];
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) pre += __shfl_xor(pre, sh, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = pre;
+    __syncthreads();
+    int start = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) start += ws[i];
+    if (last && threadIdx.x == 0) *counter = start + mine;
+    // a block without keypoints, or whose first rank is past the capacity, writes nothing
+    if (mine == 0 || start >= cap) return;  // uniform
     const int s = find_scale(sc, nscales, blk);
     const NmsScale d = sc[s];
-    const int start = offsets[blk];
-    // a block without keypoints (its count from the scan), or whose first rank is past the
-    // capacity, writes nothing
-    if (offsets[blk + 1] == start || start >= cap) return;
     const int base = (blk - d.block0) * kNmsBlockPixels;
-    constexpr int NW = kNmsThreads / 64;
     // rank of pixel (i, thread) = keypoints of earlier iterations, then of earlier waves in
     // iteration i, then of earlier lanes: the row-major order of the pixels
     __shared__ int wc[kNmsPerThread * NW];
@@ -1447,16 +1433,12 @@ hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_block
     return hipGetLastError();
 }
 
-hipError_t launch_nms_scan(int* block_counts, int total_blocks, int* counter, hipStream_t s) {
-    k_nms_scan<<<1, 1024, 0, s>>>(block_counts, total_blocks, counter);
-    return hipGetLastError();
-}
-
 hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_blocks,
-                            const int* block_offsets, const unsigned* masks, void* out, int cap,
-                            int matrix_ld, hipStream_t s) {
-    k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, block_offsets, masks, out,
-                                                     cap, matrix_ld);
+                            const int* block_counts, int* counter, const unsigned* masks,
+                            void* out, int cap, int matrix_ld, hipStream_t s) {
+    if (total_blocks < 1) return hipErrorInvalidValue;
+    k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, block_counts, total_blocks,
+                                                     counter, masks, out, cap, matrix_ld);
     return hipGetLastError();
 }
 

@@ -143,7 +143,7 @@ def test_profile_counts_every_launch(B, flags_name):
     assert count["k_pyr_down"] == frames * levels
     inv_rows = sum(v for k, v in count.items() if k.startswith("k_inv_rows"))
     assert count["k_cells"] == inv_rows and frames <= inv_rows <= frames * len(cs.ks)
-    for k in ("k_nms_count", "k_nms_scan", "k_nms_write"):
+    for k in ("k_nms_count", "k_nms_write"):
         assert count[k] == frames, k
     assert any(k.startswith("k_inv_cols") for k in prof) and any(k.startswith("k_inv_rows") for k in prof)
     for k, v in prof.items():
