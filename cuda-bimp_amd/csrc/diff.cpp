@@ -24,6 +24,10 @@
 namespace bimpk {
 hipError_t diff_prepare(int n);
 hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, hipStream_t s);
+hipError_t diff_rows_maps(float2* data, int maps, int per_map, size_t map_stride, int n,
+                          bool inv, const float2* tw, hipStream_t s);
+hipError_t diff_gather_tap_rows(const float2* Z, int kW, int kX, int FH, int FW, float2* Yc,
+                                hipStream_t s);
 hipError_t diff_cols(float2* data, int n, int w, int batch, bool inv, const float2* tw,
                      hipStream_t s);
 hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
@@ -47,7 +51,7 @@ hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_
                             hipStream_t s);
 hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, float invN, hipStream_t s);
 int diff_param_blocks(int kW);
-hipError_t diff_param_partials(const float2* Y, int kW, int kX, int FH, int FW, const float* dK,
+hipError_t diff_param_partials(const float2* Yc, int kW, int kX, int FW, const float* dK,
                                float* partials, hipStream_t s);
 hipError_t diff_fill(float* p, size_t n, hipStream_t s);
 }  // namespace bimpk
@@ -84,6 +88,8 @@ struct bimp_diff {
     std::map<int, float2*> tw;
     size_t Pmax = 0;
     float2 *Dhat = nullptr, *Z = nullptr, *acc = nullptr;
+    float2* Yc = nullptr;  // backward: 8 x kW tap rows of Y_o
+    size_t ycSize = 0;
     // forward: scales of one level and transform size share each launch (first, count)
     std::vector<std::pair<int, int>> groups;
     float2 *R = nullptr, *T = nullptr;  // forward: row spectra and inverse-column output
@@ -170,7 +176,10 @@ int scale_forward(bimp_diff* d, DScale& x, float* resp, hipStream_t s) {
     int st = fft2(d, d->Dhat, 1, g.FH, g.FW, false, s);
     if (st) return st;
     DCHK(d, bimpk::diff_spec_mul(d->Dhat, x.Q, d->Z, P, 1.0f / (float)P, s));
-    if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, true, s))) return st;
+    // inverse: columns first, then only the rows [0, Hc) the cells stencil and its adjoint
+    // read (the rest of Z and C is left column-transformed, finite, and never read)
+    DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri, true, d->tw[g.FH], s));
+    DCHK(d, bimpk::diff_rows_maps(d->Z, kNumOri, x.Hc, P, g.FW, true, d->tw[g.FW], s));
     DCHK(d, bimpk::diff_mag(d->Z, d->C, (size_t)kNumOri * P, s));
     DCHK(d, bimpk::launch_cells(d->C, g.FH, g.FW, g.FH, g.FW, g.H, g.W, x.cp, resp, d->S,
                                 d->maxslot, s));
@@ -298,6 +307,7 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
     for (auto& x : d->sc) {
         const size_t kk = (size_t)x.g.kW * x.g.kW;
         maxTaps = std::max(maxTaps, kk);
+        d->ycSize = std::max(d->ycSize, (size_t)kNumOri * x.g.kW * x.g.FW);
         d->maxBlocks = std::max(d->maxBlocks, bimpk::diff_param_blocks(x.g.kW));
         if (hipMalloc(&x.Q, (size_t)kNumOri * x.g.FH * x.g.FW * sizeof(float2)) != hipSuccess ||
             hipMalloc(&x.G, (size_t)bimpk::kUniqueOri * x.g.FH * x.g.FW * sizeof(float)) != hipSuccess ||
@@ -318,6 +328,7 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         hipMalloc(&d->R, d->rSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->T, d->tSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->Sf, d->respSize * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->Yc, d->ycSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->partials, 2 * (size_t)d->maxBlocks * d->sc.size() * sizeof(float)) != hipSuccess)
         return bail(BIMP_E_ALLOC);
     std::sort(sizes.begin(), sizes.end());
@@ -355,7 +366,7 @@ int bimp_diff_destroy(bimp_diff* d) {
     for (auto& kv : d->tw) (void)hipFree(kv.second);
     for (void* p : {(void*)d->Dhat, (void*)d->acc, (void*)d->Z, (void*)d->C,
                     (void*)d->gC, (void*)d->Wm, (void*)d->S, (void*)d->taps, (void*)d->maxslot,
-                    (void*)d->partials, (void*)d->R, (void*)d->T, (void*)d->Sf})
+                    (void*)d->partials, (void*)d->R, (void*)d->T, (void*)d->Sf, (void*)d->Yc})
         (void)hipFree(p);
     delete d;
     return BIMP_OK;
@@ -407,14 +418,19 @@ int bimp_diff_backward(bimp_diff* d, const float* d_img, int nimg, const float* 
             DCHK(d, bimpk::diff_cells_adj(d->C, g.FH, g.FW, g.H, g.W, x.cp,
                                           d_grad_resp + b * d->respSize + x.respOff, d->Wm, d->gC, s));
             DCHK(d, bimpk::diff_grad_z(d->gC, d->Z, (size_t)kNumOri * P, s));
-            if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, false, s))) return st;
+            // gZ is zero outside rows [0, Hc): their row transforms are zero, not computed
+            DCHK(d, bimpk::diff_rows_maps(d->Z, kNumOri, x.Hc, P, g.FW, false, d->tw[g.FW], s));
+            DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri, false, d->tw[g.FH], s));
             DCHK(d, bimpk::diff_accum_conjq(x.Q, d->Z, d->acc, P, invN, s));
             if ((st = fft2(d, d->acc, 1, g.FH, g.FW, true, s))) return st;
             DCHK(d, bimpk::diff_pad_adj(d->acc, g.H, g.W, g.FH, g.FW, g.kX, sc, d->glev[g.level],
                                         d->lw[g.level], s));
             DCHK(d, bimpk::diff_mul_conjd(d->Z, d->Dhat, P, invN, s));
-            if ((st = fft2(d, d->Z, kNumOri, g.FH, g.FW, true, s))) return st;
-            DCHK(d, bimpk::diff_param_partials(d->Z, g.kW, g.kX, g.FH, g.FW, x.dK,
+            // Y_o = IFFT2(...) is read at the kW x kW taps only: columns, then the kW tap rows
+            DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri, true, d->tw[g.FH], s));
+            DCHK(d, bimpk::diff_gather_tap_rows(d->Z, g.kW, g.kX, g.FH, g.FW, d->Yc, s));
+            DCHK(d, bimpk::diff_rows(d->Yc, kNumOri * g.kW, g.FW, true, d->tw[g.FW], s));
+            DCHK(d, bimpk::diff_param_partials(d->Yc, g.kW, g.kX, g.FW, x.dK,
                                                d->partials + 2 * (size_t)d->maxBlocks * si, s));
         }
         for (int l = d->maxLevel; l >= 1; --l)

@@ -28,9 +28,11 @@ using bimpdev::out_index;
 
 constexpr int kRowThreads = 256;
 
-// Batched in-place complex transforms of `rows` contiguous rows of length N.
+// Batched in-place complex transforms of rows of length N: `rows` rows in all, the first
+// `per_map` rows of each map of `map_stride` elements (per_map = rows: contiguous rows).
 template <int N, bool INV>
 __global__ void __launch_bounds__(kRowThreads) k_c2c_rows(float2* __restrict__ data, int rows,
+                                                          int per_map, size_t map_stride,
                                                           const float2* __restrict__ tw) {
     using C = FFTCfg<N>;
     constexpr int GPB = kRowThreads / C::TPV;
@@ -40,7 +42,8 @@ __global__ void __launch_bounds__(kRowThreads) k_c2c_rows(float2* __restrict__ d
     // the workgroup barrier
     const int row = blockIdx.x * GPB + grp;
     const bool act = row < rows;
-    float2* p = data + (size_t)(act ? row : 0) * N;
+    const int r = act ? row : 0;
+    float2* p = data + (size_t)(r / per_map) * map_stride + (size_t)(r % per_map) * N;
     float2 v[C::E];
 #pragma unroll
     for (int e = 0; e < C::E; ++e) v[e] = p[in_index<N>(e, lane)];
@@ -384,11 +387,22 @@ __global__ void k_mul_conjd(float2* __restrict__ G, const float2* __restrict__ D
     }
 }
 
+// Rows (r - kX) mod FH, r = 0..kW-1, of the 8 column-transformed maps Z (FH x FW each) into
+// Yc (8 x kW rows of FW): the only rows of Y_o the parameter partials read.
+__global__ void k_gather_tap_rows(const float2* __restrict__ Z, int kW, int kX, int FH, int FW,
+                                  float2* __restrict__ Yc) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, o = blockIdx.z;
+    if (x >= FW) return;
+    const int y = (r - kX + FH) % FH;
+    Yc[((size_t)o * kW + r) * FW + x] = Z[((size_t)o * FH + y) * FW + x];
+}
+
 // Parameter gradient partials: sum over orientations and taps of
-//   Re Y_o(tap) dKe/dtheta + Im Y_o(tap) dKo/dtheta  for theta = (sigma ratio, gamma);
-// one pair of partial sums per block (fixed-order tree), summed on the host in block order.
-__global__ void __launch_bounds__(256) k_param_partials(const float2* __restrict__ Y, int kW,
-                                                        int kX, int FH, int FW,
+//   Re Y_o(tap) dKe/dtheta + Im Y_o(tap) dKo/dtheta  for theta = (sigma ratio, gamma),
+// with Y_o's tap rows in Yc (k_gather_tap_rows, then row transforms); one pair of partial
+// sums per block (fixed-order tree), summed on the host in block order.
+__global__ void __launch_bounds__(256) k_param_partials(const float2* __restrict__ Yc, int kW,
+                                                        int kX, int FW,
                                                         const float* __restrict__ dK,
                                                         float* __restrict__ partials) {
     __shared__ float sr[256], sg[256];
@@ -397,8 +411,8 @@ __global__ void __launch_bounds__(256) k_param_partials(const float2* __restrict
     float vr = 0.f, vg = 0.f;
     if (i < n) {
         const int o = i / (kW * kW), m = i % (kW * kW), r = m / kW, c = m % kW;
-        const int y = (r - kX + FH) % FH, x = (c - kX + FW) % FW;
-        const float2 v = Y[((size_t)o * FH + y) * FW + x];
+        const int x = (c - kX + FW) % FW;
+        const float2 v = Yc[((size_t)o * kW + r) * FW + x];
         const size_t stride = (size_t)n;
         vr = v.x * dK[0 * stride + i] + v.y * dK[1 * stride + i];
         vg = v.x * dK[2 * stride + i] + v.y * dK[3 * stride + i];
@@ -451,21 +465,30 @@ hipError_t diff_prepare(int n) {
     return hipErrorInvalidValue;
 }
 
-hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, hipStream_t s) {
+hipError_t diff_rows_maps(float2* data, int maps, int per_map, size_t map_stride, int n,
+                          bool inv, const float2* tw, hipStream_t s) {
+    const int rows = maps * per_map;
+    if (maps < 1 || per_map < 1) return hipErrorInvalidValue;
 #define X(N)                                                                                 \
     if (n == N) {                                                                            \
         constexpr int GPB = kRowThreads / FFTCfg<N>::TPV;                                    \
         const int lds = GPB * bimpdev::fft_lds_floats<N>() * (int)sizeof(float);             \
         dim3 grid((rows + GPB - 1) / GPB);                                                   \
         if (inv)                                                                             \
-            k_c2c_rows<N, true><<<grid, kRowThreads, lds, s>>>(data, rows, tw);              \
+            k_c2c_rows<N, true><<<grid, kRowThreads, lds, s>>>(data, rows, per_map, map_stride, \
+                                                               tw);                          \
         else                                                                                 \
-            k_c2c_rows<N, false><<<grid, kRowThreads, lds, s>>>(data, rows, tw);             \
+            k_c2c_rows<N, false><<<grid, kRowThreads, lds, s>>>(data, rows, per_map,         \
+                                                                map_stride, tw);             \
         return hipGetLastError();                                                            \
     }
     BIMP_FOR_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, hipStream_t s) {
+    return diff_rows_maps(data, 1, rows, 0, n, inv, tw, s);
 }
 
 hipError_t diff_cols(float2* data, int n, int w, int batch, bool inv, const float2* tw,
@@ -559,9 +582,15 @@ hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, float invN, hipS
 
 int diff_param_blocks(int kW) { return (kNumOri * kW * kW + 255) / 256; }
 
-hipError_t diff_param_partials(const float2* Y, int kW, int kX, int FH, int FW, const float* dK,
+hipError_t diff_gather_tap_rows(const float2* Z, int kW, int kX, int FH, int FW, float2* Yc,
+                                hipStream_t s) {
+    k_gather_tap_rows<<<dim3((FW + 127) / 128, kW, kNumOri), 128, 0, s>>>(Z, kW, kX, FH, FW, Yc);
+    return hipGetLastError();
+}
+
+hipError_t diff_param_partials(const float2* Yc, int kW, int kX, int FW, const float* dK,
                                float* partials, hipStream_t s) {
-    k_param_partials<<<diff_param_blocks(kW), 256, 0, s>>>(Y, kW, kX, FH, FW, dK, partials);
+    k_param_partials<<<diff_param_blocks(kW), 256, 0, s>>>(Yc, kW, kX, FW, dK, partials);
     return hipGetLastError();
 }
 
