@@ -27,32 +27,34 @@ hipError_t diff_rows(float2* data, int rows, int n, bool inv, const float2* tw, 
 hipError_t diff_rows_maps(float2* data, int maps, int per_map, size_t map_stride, int n,
                           bool inv, const float2* tw, hipStream_t s);
 hipError_t diff_gather_tap_rows(const float2* Z, int kW, int kX, int FH, int FW, float2* Yc,
-                                hipStream_t s);
+                                int nb, hipStream_t s);
 hipError_t diff_cols(float2* data, int n, int w, int batch, bool inv, const float2* tw,
                      hipStream_t s);
-hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
+hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh, int nb,
                          hipStream_t s);
 hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int sw, int sh,
-                             hipStream_t s);
-hipError_t diff_pad(const float* L, int pitch, int H, int W, int FH, int FW, int k, float sc,
-                    float2* out, hipStream_t s);
+                             int nb, hipStream_t s);
+hipError_t diff_pad(const float* L, int pitch, size_t lstride, int H, int W, int FH, int FW,
+                    int k, float sc, float2* out, int nb, hipStream_t s);
 hipError_t diff_pad_adj(const float2* g, int H, int W, int FH, int FW, int k, float sc,
-                        float* gL, int pitch, hipStream_t s);
+                        float* gL, int pitch, size_t lstride, int nb, hipStream_t s);
 hipError_t diff_q_to_g(const float2* Q, int FH, int FW, float invN, float* G, hipStream_t s);
 hipError_t diff_bank_pad(const float* ke, const float* ko, int kW, int kX, int FH, int FW,
                          float2* out, hipStream_t s);
-hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, float invN,
-                         hipStream_t s);
+hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, int nb,
+                         float invN, hipStream_t s);
 hipError_t diff_mag(const float2* Z, float* C, size_t n, hipStream_t s);
 hipError_t diff_cells_adj(const float* C, int FH, int FW, int H, int W, const CellParams& p,
-                          const float* gD, float* Wm, float* gC, hipStream_t s);
+                          const float* gD, size_t gstride, float* Wm, float* gC, int nb,
+                          hipStream_t s);
 hipError_t diff_grad_z(const float* gC, float2* Z, size_t n, hipStream_t s);
-hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_t n, float invN,
-                            hipStream_t s);
-hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, float invN, hipStream_t s);
+hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_t n, int nb,
+                            float invN, hipStream_t s);
+hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, int nb, float invN,
+                          hipStream_t s);
 int diff_param_blocks(int kW);
 hipError_t diff_param_partials(const float2* Yc, int kW, int kX, int FW, const float* dK,
-                               float* partials, hipStream_t s);
+                               float* partials, size_t pstride, int nb, hipStream_t s);
 hipError_t diff_fill(float* p, size_t n, hipStream_t s);
 }  // namespace bimpk
 
@@ -99,6 +101,7 @@ struct bimp_diff {
     unsigned* maxslot = nullptr;
     float* partials = nullptr;
     int maxBlocks = 0;
+    int nbMax = 1;  // images per backward launch (the per-image buffers hold that many)
     std::string err;
 };
 
@@ -156,33 +159,41 @@ int set_params(bimp_diff* d, float ratio, float gamma, hipStream_t s) {
 }
 
 // Float pyramid of one image into d->lev.
-int pyramid(bimp_diff* d, const float* img, hipStream_t s) {
-    DCHK(d, hipMemcpyAsync(d->lev[0], img, (size_t)d->width * d->height * 4,
+// Float pyramids of nb consecutive images into d->lev (level l of image b at
+// lev[l] + b * lw[l] * lh[l]).
+int pyramid(bimp_diff* d, const float* img, int nb, hipStream_t s) {
+    DCHK(d, hipMemcpyAsync(d->lev[0], img, (size_t)nb * d->width * d->height * 4,
                            hipMemcpyDeviceToDevice, s));
     for (int l = 1; l <= d->maxLevel; ++l)
         DCHK(d, bimpk::diff_pyr_down(d->lev[l - 1], d->lw[l - 1], d->lh[l - 1], d->lev[l], d->lw[l],
-                                     d->lh[l], s));
+                                     d->lh[l], nb, s));
     return BIMP_OK;
 }
 
-// Forward of one scale for the backward pass: Dhat, Z, C on the whole FH x FW domain, and
-// the D map into `resp`.
-int scale_forward(bimp_diff* d, DScale& x, float* resp, hipStream_t s) {
+// Forward of one scale for the backward pass, for the nb images whose pyramids are in
+// d->lev: Dhat, Z, C per image on the whole FH x FW domain (Z and C valid in rows [0, Hc)),
+// and the D maps into scratch.
+int scale_forward(bimp_diff* d, DScale& x, int nb, hipStream_t s) {
     const auto& g = x.g;
     const size_t P = (size_t)g.FH * g.FW;
     const float sc = 255.0f * d->opt.input_scale;
-    DCHK(d, bimpk::diff_pad(d->lev[g.level], d->lw[g.level], g.H, g.W, g.FH, g.FW, g.kX, sc,
-                            d->Dhat, s));
-    int st = fft2(d, d->Dhat, 1, g.FH, g.FW, false, s);
+    DCHK(d, bimpk::diff_pad(d->lev[g.level], d->lw[g.level],
+                            (size_t)d->lw[g.level] * d->lh[g.level], g.H, g.W, g.FH, g.FW, g.kX,
+                            sc, d->Dhat, nb, s));
+    int st = fft2(d, d->Dhat, nb, g.FH, g.FW, false, s);
     if (st) return st;
-    DCHK(d, bimpk::diff_spec_mul(d->Dhat, x.Q, d->Z, P, 1.0f / (float)P, s));
+    DCHK(d, bimpk::diff_spec_mul(d->Dhat, x.Q, d->Z, P, nb, 1.0f / (float)P, s));
     // inverse: columns first, then only the rows [0, Hc) the cells stencil and its adjoint
     // read (the rest of Z and C is left column-transformed, finite, and never read)
-    DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri, true, d->tw[g.FH], s));
-    DCHK(d, bimpk::diff_rows_maps(d->Z, kNumOri, x.Hc, P, g.FW, true, d->tw[g.FW], s));
-    DCHK(d, bimpk::diff_mag(d->Z, d->C, (size_t)kNumOri * P, s));
-    DCHK(d, bimpk::launch_cells(d->C, g.FH, g.FW, g.FH, g.FW, g.H, g.W, x.cp, resp, d->S,
-                                d->maxslot, s));
+    DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri * nb, true, d->tw[g.FH], s));
+    DCHK(d, bimpk::diff_rows_maps(d->Z, kNumOri * nb, x.Hc, P, g.FW, true, d->tw[g.FW], s));
+    DCHK(d, bimpk::diff_mag(d->Z, d->C, (size_t)kNumOri * P * nb, s));
+    bimpk::CellScale cs[bimpk::kCellMaxScales];
+    for (int b = 0; b < nb; ++b)
+        cs[b] = bimpk::CellScale{d->C + (size_t)b * kNumOri * P, g.FH, g.FW, g.FH, g.FW, g.H,
+                                 g.W, x.cp, d->S + (size_t)b * 2 * P + P,
+                                 d->S + (size_t)b * 2 * P, d->maxslot};
+    DCHK(d, bimpk::launch_cells(cs, nb, s));
     return BIMP_OK;
 }
 
@@ -296,11 +307,22 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         d->lw[l] = (d->lw[l - 1] + 1) / 2;
         d->lh[l] = (d->lh[l - 1] + 1) / 2;
     }
+    // images per backward launch: up to the cells launch's 8 jobs, while the per-image
+    // buffers (8 complex and 44 real maps of the largest transform) stay within 4 GB
+    {
+        size_t pm = 0;
+        for (const auto& x : d->sc) pm = std::max(pm, (size_t)x.g.FH * x.g.FW);
+        const size_t per = pm * (2 * sizeof(float2) + kNumOri * sizeof(float2) +
+                                 (2 + 4) * kNumOri * sizeof(float) + 2 * sizeof(float));
+        d->nbMax = (int)std::max<size_t>(1, std::min<size_t>(bimpk::kCellMaxScales,
+                                                             (size_t(4) << 30) / per));
+    }
+    const size_t NB = (size_t)d->nbMax;
     d->lev.assign(d->maxLevel + 1, nullptr);
     d->glev.assign(d->maxLevel + 1, nullptr);
     for (int l = 0; l <= d->maxLevel; ++l)
-        if (hipMalloc(&d->lev[l], (size_t)d->lw[l] * d->lh[l] * 4) != hipSuccess ||
-            hipMalloc(&d->glev[l], (size_t)d->lw[l] * d->lh[l] * 4) != hipSuccess)
+        if (hipMalloc(&d->lev[l], NB * d->lw[l] * d->lh[l] * 4) != hipSuccess ||
+            hipMalloc(&d->glev[l], NB * d->lw[l] * d->lh[l] * 4) != hipSuccess)
             return bail(BIMP_E_ALLOC);
     const size_t P = d->Pmax;
     size_t maxTaps = 0;
@@ -316,20 +338,21 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
             hipMalloc(&x.dK, 4 * kNumOri * kk * sizeof(float)) != hipSuccess)
             return bail(BIMP_E_ALLOC);
     }
-    if (hipMalloc(&d->Dhat, P * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&d->acc, P * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&d->Z, kNumOri * P * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&d->C, kNumOri * P * sizeof(float)) != hipSuccess ||
-        hipMalloc(&d->gC, kNumOri * P * sizeof(float)) != hipSuccess ||
-        hipMalloc(&d->Wm, 4 * kNumOri * P * sizeof(float)) != hipSuccess ||
-        hipMalloc(&d->S, 2 * P * sizeof(float)) != hipSuccess ||
+    if (hipMalloc(&d->Dhat, NB * P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->acc, NB * P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->Z, NB * kNumOri * P * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->C, NB * kNumOri * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->gC, NB * kNumOri * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->Wm, NB * 4 * kNumOri * P * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->S, NB * 2 * P * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->taps, 2 * kNumOri * maxTaps * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->maxslot, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&d->R, d->rSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->T, d->tSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->Sf, d->respSize * sizeof(float)) != hipSuccess ||
-        hipMalloc(&d->Yc, d->ycSize * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&d->partials, 2 * (size_t)d->maxBlocks * d->sc.size() * sizeof(float)) != hipSuccess)
+        hipMalloc(&d->Yc, NB * d->ycSize * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&d->partials, NB * 2 * (size_t)d->maxBlocks * d->sc.size() * sizeof(float)) !=
+                hipSuccess)
         return bail(BIMP_E_ALLOC);
     std::sort(sizes.begin(), sizes.end());
     sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
@@ -386,7 +409,7 @@ int bimp_diff_forward(bimp_diff* d, const float* d_img, int nimg, float* d_resp,
     hipStream_t s = (hipStream_t)stream;
     const size_t isz = (size_t)d->width * d->height;
     for (int b = 0; b < nimg; ++b) {
-        int st = pyramid(d, d_img + b * isz, s);
+        int st = pyramid(d, d_img + b * isz, 1, s);
         if (st) return st;
         for (const auto& gr : d->groups)
             if ((st = group_forward(d, gr.first, gr.second, d_resp + b * d->respSize, s)))
@@ -403,50 +426,60 @@ int bimp_diff_backward(bimp_diff* d, const float* d_img, int nimg, const float* 
     const float sc = 255.0f * d->opt.input_scale;
     double gr = 0, gg = 0;
     std::vector<float> part;
-    for (int b = 0; b < nimg; ++b) {
-        int st = pyramid(d, d_img + b * isz, s);
+    const size_t pstride = 2 * (size_t)d->maxBlocks * d->sc.size();  // partials per image
+    // chunks of up to nbMax images: every launch below covers the chunk's images at once
+    for (int b0 = 0; b0 < nimg; b0 += d->nbMax) {
+        const int nb = std::min(d->nbMax, nimg - b0);
+        int st = pyramid(d, d_img + b0 * isz, nb, s);
         if (st) return st;
         for (int l = 0; l <= d->maxLevel; ++l)
-            DCHK(d, bimpk::diff_fill(d->glev[l], (size_t)d->lw[l] * d->lh[l], s));
+            DCHK(d, bimpk::diff_fill(d->glev[l], (size_t)nb * d->lw[l] * d->lh[l], s));
         for (size_t si = 0; si < d->sc.size(); ++si) {
             DScale& x = d->sc[si];
             const auto& g = x.g;
             const size_t P = (size_t)g.FH * g.FW;
             const float invN = 1.0f / (float)P;
-            // recompute the responses (the D map goes to scratch)
-            if ((st = scale_forward(d, x, d->S + P, s))) return st;
+            // recompute the responses (the D maps go to scratch)
+            if ((st = scale_forward(d, x, nb, s))) return st;
             DCHK(d, bimpk::diff_cells_adj(d->C, g.FH, g.FW, g.H, g.W, x.cp,
-                                          d_grad_resp + b * d->respSize + x.respOff, d->Wm, d->gC, s));
-            DCHK(d, bimpk::diff_grad_z(d->gC, d->Z, (size_t)kNumOri * P, s));
+                                          d_grad_resp + b0 * d->respSize + x.respOff, d->respSize,
+                                          d->Wm, d->gC, nb, s));
+            DCHK(d, bimpk::diff_grad_z(d->gC, d->Z, (size_t)kNumOri * P * nb, s));
             // gZ is zero outside rows [0, Hc): their row transforms are zero, not computed
-            DCHK(d, bimpk::diff_rows_maps(d->Z, kNumOri, x.Hc, P, g.FW, false, d->tw[g.FW], s));
-            DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri, false, d->tw[g.FH], s));
-            DCHK(d, bimpk::diff_accum_conjq(x.Q, d->Z, d->acc, P, invN, s));
-            if ((st = fft2(d, d->acc, 1, g.FH, g.FW, true, s))) return st;
+            DCHK(d, bimpk::diff_rows_maps(d->Z, kNumOri * nb, x.Hc, P, g.FW, false, d->tw[g.FW], s));
+            DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri * nb, false, d->tw[g.FH], s));
+            DCHK(d, bimpk::diff_accum_conjq(x.Q, d->Z, d->acc, P, nb, invN, s));
+            if ((st = fft2(d, d->acc, nb, g.FH, g.FW, true, s))) return st;
             DCHK(d, bimpk::diff_pad_adj(d->acc, g.H, g.W, g.FH, g.FW, g.kX, sc, d->glev[g.level],
-                                        d->lw[g.level], s));
-            DCHK(d, bimpk::diff_mul_conjd(d->Z, d->Dhat, P, invN, s));
+                                        d->lw[g.level], (size_t)d->lw[g.level] * d->lh[g.level],
+                                        nb, s));
+            DCHK(d, bimpk::diff_mul_conjd(d->Z, d->Dhat, P, nb, invN, s));
             // Y_o = IFFT2(...) is read at the kW x kW taps only: columns, then the kW tap rows
-            DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri, true, d->tw[g.FH], s));
-            DCHK(d, bimpk::diff_gather_tap_rows(d->Z, g.kW, g.kX, g.FH, g.FW, d->Yc, s));
-            DCHK(d, bimpk::diff_rows(d->Yc, kNumOri * g.kW, g.FW, true, d->tw[g.FW], s));
+            DCHK(d, bimpk::diff_cols(d->Z, g.FH, g.FW, kNumOri * nb, true, d->tw[g.FH], s));
+            DCHK(d, bimpk::diff_gather_tap_rows(d->Z, g.kW, g.kX, g.FH, g.FW, d->Yc, nb, s));
+            DCHK(d, bimpk::diff_rows(d->Yc, kNumOri * g.kW * nb, g.FW, true, d->tw[g.FW], s));
             DCHK(d, bimpk::diff_param_partials(d->Yc, g.kW, g.kX, g.FW, x.dK,
-                                               d->partials + 2 * (size_t)d->maxBlocks * si, s));
+                                               d->partials + 2 * (size_t)d->maxBlocks * si,
+                                               pstride, nb, s));
         }
         for (int l = d->maxLevel; l >= 1; --l)
             DCHK(d, bimpk::diff_pyr_down_adj(d->glev[l], d->lw[l], d->lh[l], d->glev[l - 1],
-                                             d->lw[l - 1], d->lh[l - 1], s));
-        DCHK(d, hipMemcpyAsync(d_grad_img + b * isz, d->glev[0], isz * 4, hipMemcpyDeviceToDevice, s));
-        part.resize(2 * (size_t)d->maxBlocks * d->sc.size());
+                                             d->lw[l - 1], d->lh[l - 1], nb, s));
+        DCHK(d, hipMemcpyAsync(d_grad_img + b0 * isz, d->glev[0], (size_t)nb * isz * 4,
+                               hipMemcpyDeviceToDevice, s));
+        part.resize(pstride * nb);
         DCHK(d, hipMemcpyAsync(part.data(), d->partials, part.size() * 4, hipMemcpyDeviceToHost, s));
         DCHK(d, hipStreamSynchronize(s));
-        for (size_t si = 0; si < d->sc.size(); ++si) {
-            const int nbk = bimpk::diff_param_blocks(d->sc[si].g.kW);
-            for (int k = 0; k < nbk; ++k) {
-                gr += part[2 * ((size_t)d->maxBlocks * si + k)];
-                gg += part[2 * ((size_t)d->maxBlocks * si + k) + 1];
+        // image by image, scale by scale, block by block: the order of one image at a time
+        for (int b = 0; b < nb; ++b)
+            for (size_t si = 0; si < d->sc.size(); ++si) {
+                const int nbk = bimpk::diff_param_blocks(d->sc[si].g.kW);
+                const float* pp = part.data() + pstride * b + 2 * (size_t)d->maxBlocks * si;
+                for (int k = 0; k < nbk; ++k) {
+                    gr += pp[2 * k];
+                    gg += pp[2 * k + 1];
+                }
             }
-        }
     }
     if (grad_params) {
         grad_params[0] = gr;

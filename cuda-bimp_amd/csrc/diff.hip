@@ -103,10 +103,13 @@ __device__ __forceinline__ int reflect101(int t, int n) {
 }
 
 // Float pyramid level (the 5x5 binomial / 256 of pyrDown with reflect-101 borders, no rounding).
+// blockIdx.z: image of the batch (levels of one size back to back)
 __global__ void k_pyr_down_f(const float* __restrict__ src, int sw, int sh,
                              float* __restrict__ dst, int dw, int dh) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= dw) return;
+    src += (size_t)blockIdx.z * sw * sh;
+    dst += (size_t)blockIdx.z * dw * dh;
     const float k[5] = {1.f, 4.f, 6.f, 4.f, 1.f};
     float acc = 0.f;
     for (int i = -2; i <= 2; ++i) {
@@ -148,6 +151,8 @@ __global__ void k_pyr_down_adj(const float* __restrict__ gdst, int dw, int dh,
                                float* __restrict__ gsrc, int sw, int sh) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= sw) return;
+    gdst += (size_t)blockIdx.z * dw * dh;
+    gsrc += (size_t)blockIdx.z * sw * sh;
     int dy[9], dx[9];
     float wy[9], wx[9];
     const int ny = pyr_taps(y, sh, dh, dy, wy);
@@ -165,10 +170,13 @@ __device__ __forceinline__ int clamp_src(int y, int H, int k) { return y < H ? y
 
 // Pd = s * pad(L) as complex (imaginary part 0).  L is the top-left H x W region of a level
 // image with `pitch` floats per row (a level is (n+1)/2 wide, its data n/2).
-__global__ void k_pad_fwd(const float* __restrict__ L, int pitch, int H, int W, int FH, int FW,
-                          int k, float s, float2* __restrict__ out) {
+// blockIdx.z: image of the batch (level images lstride floats apart, outputs FH*FW apart)
+__global__ void k_pad_fwd(const float* __restrict__ L, int pitch, size_t lstride, int H, int W,
+                          int FH, int FW, int k, float s, float2* __restrict__ out) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= FW) return;
+    L += blockIdx.z * lstride;
+    out += (size_t)blockIdx.z * FH * FW;
     out[(size_t)y * FW + x] =
             make_float2(s * L[(size_t)clamp_src(y, H, k) * pitch + clamp_src(x, W, k)], 0.f);
 }
@@ -189,9 +197,11 @@ __device__ __forceinline__ void pad_band(int r, int H, int F, int k, int* lo1, i
 
 // gL += s * pad^T(Re gPd) over the top-left H x W region of gL (`pitch` floats per row)
 __global__ void k_pad_adj(const float2* __restrict__ g, int H, int W, int FH, int FW, int k,
-                          float s, float* __restrict__ gL, int pitch) {
+                          float s, float* __restrict__ gL, int pitch, size_t lstride) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W) return;
+    g += (size_t)blockIdx.z * FH * FW;
+    gL += blockIdx.z * lstride;
     int ylo, yhi, xlo, xhi;
     pad_band(y, H, FH, k, &ylo, &yhi);
     pad_band(x, W, FW, k, &xlo, &xhi);
@@ -241,16 +251,18 @@ __global__ void k_bank_pad(const float* __restrict__ ke, const float* __restrict
     out[((size_t)o * FH + y) * FW + x] = v;
 }
 
-// Z_o = D * Q_o * invN for the 8 orientations.
+// Z_o = D * Q_o * invN for the 8 orientations; nb images (D: nb maps of n, Z: nb x 8 maps).
 __global__ void k_spec_mul(const float2* __restrict__ D, const float2* __restrict__ Q,
-                           float2* __restrict__ Z, size_t n, float invN) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float2 d = D[i];
+                           float2* __restrict__ Z, size_t n, int nb, float invN) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nb) return;
+    const size_t b = t / n, i = t - b * n;
+    const float2 d = D[t];
+    float2* z = Z + b * kNumOri * n;
 #pragma unroll
     for (int o = 0; o < kNumOri; ++o) {
         const float2 q = Q[(size_t)o * n + i];
-        Z[(size_t)o * n + i] =
+        z[(size_t)o * n + i] =
             make_float2((d.x * q.x - d.y * q.y) * invN, (d.x * q.y + d.y * q.x) * invN);
     }
 }
@@ -279,11 +291,16 @@ __device__ __forceinline__ bool valid(int x, int y, int a, int b, int W, int H) 
 // active-branch masks of the forward (cuda_bimp.cuh:436-502):
 //   Wd = g [ds > 0], Wt = g [tangential > 0], W1 = g [v2 > 0][v1 > 0], W2 = g [v2 > 0]
 // stored as 4 x 8 maps over the FH x FW grid (zero outside the data region).
+// blockIdx.z: image of the batch (C: 8 maps, Wm: 32 maps per image; gD gstride apart)
 __global__ void k_cells_adj_w(const float* __restrict__ C, int FH, int FW, int H, int W,
-                              CellParams p, const float* __restrict__ gD, float* __restrict__ Wm) {
+                              CellParams p, const float* __restrict__ gD, size_t gstride,
+                              float* __restrict__ Wm) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= FW) return;
     const long P = (long)FH * FW, g0 = (long)y * FW + x;
+    C += (size_t)blockIdx.z * kNumOri * P;
+    gD += blockIdx.z * gstride;
+    Wm += (size_t)blockIdx.z * 4 * kNumOri * P;
     const float g = (y < H && x < W) ? gD[(size_t)y * W + x] : 0.f;
 #pragma unroll
     for (int o = 0; o < kNumOri; ++o) {
@@ -323,11 +340,15 @@ __global__ void k_cells_adj_w(const float* __restrict__ C, int FH, int FW, int H
 // Pass 2: gather, for each map position p and orientation o, every forward read of C_o(p):
 //   D = sum_o relu(C(x) - C(x+d1)/2 - C(x+d2)/2) - sum_o [relu(C(x+t) - C(x)) + R_o(x)],
 //   R_o(x) = relu(relu(C_o(x) - 16 C_o'(x+r)) + C_o(x) - 16 C_o'(x-r)), o' = o+4.
+// blockIdx.z = image * 8 + orientation
 __global__ void k_cells_adj_g(const float* __restrict__ Wm, int FH, int FW, CellParams p,
                               float* __restrict__ gC) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, o = blockIdx.z;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    const int o = blockIdx.z % kNumOri, img = blockIdx.z / kNumOri;
     if (x >= FW) return;
     const long P = (long)FH * FW, q = (long)y * FW + x;
+    Wm += (size_t)img * 4 * kNumOri * P;
+    gC += (size_t)img * kNumOri * P;
     const float* Wd = Wm + ((size_t)0 * kNumOri + o) * P;
     const float* Wt = Wm + ((size_t)1 * kNumOri + o) * P;
     const int a = p.ea[o], b = p.eb[o];
@@ -358,31 +379,35 @@ __global__ void k_grad_z(const float* __restrict__ gC, float2* __restrict__ Z, s
     Z[i] = make_float2(z.x * s, z.y * s);
 }
 
-// out = invN * sum_o conj(Q_o) * G_o
+// out = invN * sum_o conj(Q_o) * G_o per image (G: nb x 8 maps of n, out: nb maps)
 __global__ void k_accum_conjq(const float2* __restrict__ Q, const float2* __restrict__ G,
-                              float2* __restrict__ out, size_t n, float invN) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                              float2* __restrict__ out, size_t n, int nb, float invN) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nb) return;
+    const size_t b = t / n, i = t - b * n;
+    const float2* gb = G + b * kNumOri * n;
     float re = 0.f, im = 0.f;
 #pragma unroll
     for (int o = 0; o < kNumOri; ++o) {
-        const float2 q = Q[(size_t)o * n + i], g = G[(size_t)o * n + i];
+        const float2 q = Q[(size_t)o * n + i], g = gb[(size_t)o * n + i];
         re += q.x * g.x + q.y * g.y;
         im += q.x * g.y - q.y * g.x;
     }
-    out[i] = make_float2(re * invN, im * invN);
+    out[t] = make_float2(re * invN, im * invN);
 }
 
-// G_o = invN * G_o * conj(D) in place.
+// G_o = invN * G_o * conj(D) in place, per image (G: nb x 8 maps of n, D: nb maps).
 __global__ void k_mul_conjd(float2* __restrict__ G, const float2* __restrict__ D, size_t n,
-                            float invN) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float2 d = D[i];
+                            int nb, float invN) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nb) return;
+    const size_t b = t / n, i = t - b * n;
+    const float2 d = D[t];
+    float2* gb = G + b * kNumOri * n;
 #pragma unroll
     for (int o = 0; o < kNumOri; ++o) {
-        const float2 g = G[(size_t)o * n + i];
-        G[(size_t)o * n + i] =
+        const float2 g = gb[(size_t)o * n + i];
+        gb[(size_t)o * n + i] =
             make_float2((g.x * d.x + g.y * d.y) * invN, (g.y * d.x - g.x * d.y) * invN);
     }
 }
@@ -404,10 +429,14 @@ __global__ void k_gather_tap_rows(const float2* __restrict__ Z, int kW, int kX, 
 __global__ void __launch_bounds__(256) k_param_partials(const float2* __restrict__ Yc, int kW,
                                                         int kX, int FW,
                                                         const float* __restrict__ dK,
-                                                        float* __restrict__ partials) {
+                                                        float* __restrict__ partials,
+                                                        size_t pstride) {
     __shared__ float sr[256], sg[256];
     const int n = kNumOri * kW * kW;
     const int i = blockIdx.x * 256 + threadIdx.x;
+    // blockIdx.y: image of the batch (its tap rows and its partials)
+    Yc += (size_t)blockIdx.y * kNumOri * kW * FW;
+    partials += (size_t)blockIdx.y * pstride;
     float vr = 0.f, vg = 0.f;
     if (i < n) {
         const int o = i / (kW * kW), m = i % (kW * kW), r = m / kW, c = m % kW;
@@ -509,27 +538,29 @@ hipError_t diff_cols(float2* data, int n, int w, int batch, bool inv, const floa
     return hipErrorInvalidValue;
 }
 
-hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh,
+hipError_t diff_pyr_down(const float* src, int sw, int sh, float* dst, int dw, int dh, int nb,
                          hipStream_t s) {
-    k_pyr_down_f<<<dim3((dw + 127) / 128, dh), 128, 0, s>>>(src, sw, sh, dst, dw, dh);
+    k_pyr_down_f<<<dim3((dw + 127) / 128, dh, nb), 128, 0, s>>>(src, sw, sh, dst, dw, dh);
     return hipGetLastError();
 }
 
 hipError_t diff_pyr_down_adj(const float* gdst, int dw, int dh, float* gsrc, int sw, int sh,
-                             hipStream_t s) {
-    k_pyr_down_adj<<<dim3((sw + 127) / 128, sh), 128, 0, s>>>(gdst, dw, dh, gsrc, sw, sh);
+                             int nb, hipStream_t s) {
+    k_pyr_down_adj<<<dim3((sw + 127) / 128, sh, nb), 128, 0, s>>>(gdst, dw, dh, gsrc, sw, sh);
     return hipGetLastError();
 }
 
-hipError_t diff_pad(const float* L, int pitch, int H, int W, int FH, int FW, int k, float sc,
-                    float2* out, hipStream_t s) {
-    k_pad_fwd<<<dim3((FW + 127) / 128, FH), 128, 0, s>>>(L, pitch, H, W, FH, FW, k, sc, out);
+hipError_t diff_pad(const float* L, int pitch, size_t lstride, int H, int W, int FH, int FW,
+                    int k, float sc, float2* out, int nb, hipStream_t s) {
+    k_pad_fwd<<<dim3((FW + 127) / 128, FH, nb), 128, 0, s>>>(L, pitch, lstride, H, W, FH, FW, k,
+                                                             sc, out);
     return hipGetLastError();
 }
 
 hipError_t diff_pad_adj(const float2* g, int H, int W, int FH, int FW, int k, float sc,
-                        float* gL, int pitch, hipStream_t s) {
-    k_pad_adj<<<dim3((W + 127) / 128, H), 128, 0, s>>>(g, H, W, FH, FW, k, sc, gL, pitch);
+                        float* gL, int pitch, size_t lstride, int nb, hipStream_t s) {
+    k_pad_adj<<<dim3((W + 127) / 128, H, nb), 128, 0, s>>>(g, H, W, FH, FW, k, sc, gL, pitch,
+                                                           lstride);
     return hipGetLastError();
 }
 
@@ -544,9 +575,9 @@ hipError_t diff_bank_pad(const float* ke, const float* ko, int kW, int kX, int F
     return hipGetLastError();
 }
 
-hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, float invN,
-                         hipStream_t s) {
-    k_spec_mul<<<ew(n), 256, 0, s>>>(D, Q, Z, n, invN);
+hipError_t diff_spec_mul(const float2* D, const float2* Q, float2* Z, size_t n, int nb,
+                         float invN, hipStream_t s) {
+    k_spec_mul<<<ew(n * nb), 256, 0, s>>>(D, Q, Z, n, nb, invN);
     return hipGetLastError();
 }
 
@@ -556,11 +587,13 @@ hipError_t diff_mag(const float2* Z, float* C, size_t n, hipStream_t s) {
 }
 
 hipError_t diff_cells_adj(const float* C, int FH, int FW, int H, int W, const CellParams& p,
-                          const float* gD, float* Wm, float* gC, hipStream_t s) {
-    k_cells_adj_w<<<dim3((FW + 127) / 128, FH), 128, 0, s>>>(C, FH, FW, H, W, p, gD, Wm);
+                          const float* gD, size_t gstride, float* Wm, float* gC, int nb,
+                          hipStream_t s) {
+    k_cells_adj_w<<<dim3((FW + 127) / 128, FH, nb), 128, 0, s>>>(C, FH, FW, H, W, p, gD, gstride,
+                                                                 Wm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_cells_adj_g<<<dim3((FW + 127) / 128, FH, kNumOri), 128, 0, s>>>(Wm, FH, FW, p, gC);
+    k_cells_adj_g<<<dim3((FW + 127) / 128, FH, kNumOri * nb), 128, 0, s>>>(Wm, FH, FW, p, gC);
     return hipGetLastError();
 }
 
@@ -569,28 +602,32 @@ hipError_t diff_grad_z(const float* gC, float2* Z, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_t n, float invN,
-                            hipStream_t s) {
-    k_accum_conjq<<<ew(n), 256, 0, s>>>(Q, G, out, n, invN);
+hipError_t diff_accum_conjq(const float2* Q, const float2* G, float2* out, size_t n, int nb,
+                            float invN, hipStream_t s) {
+    k_accum_conjq<<<ew(n * nb), 256, 0, s>>>(Q, G, out, n, nb, invN);
     return hipGetLastError();
 }
 
-hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, float invN, hipStream_t s) {
-    k_mul_conjd<<<ew(n), 256, 0, s>>>(G, D, n, invN);
+hipError_t diff_mul_conjd(float2* G, const float2* D, size_t n, int nb, float invN,
+                          hipStream_t s) {
+    k_mul_conjd<<<ew(n * nb), 256, 0, s>>>(G, D, n, nb, invN);
     return hipGetLastError();
 }
 
 int diff_param_blocks(int kW) { return (kNumOri * kW * kW + 255) / 256; }
 
 hipError_t diff_gather_tap_rows(const float2* Z, int kW, int kX, int FH, int FW, float2* Yc,
-                                hipStream_t s) {
-    k_gather_tap_rows<<<dim3((FW + 127) / 128, kW, kNumOri), 128, 0, s>>>(Z, kW, kX, FH, FW, Yc);
+                                int nb, hipStream_t s) {
+    // map index z = image * 8 + orientation in both Z and Yc
+    k_gather_tap_rows<<<dim3((FW + 127) / 128, kW, kNumOri * nb), 128, 0, s>>>(Z, kW, kX, FH, FW,
+                                                                               Yc);
     return hipGetLastError();
 }
 
 hipError_t diff_param_partials(const float2* Yc, int kW, int kX, int FW, const float* dK,
-                               float* partials, hipStream_t s) {
-    k_param_partials<<<diff_param_blocks(kW), 256, 0, s>>>(Yc, kW, kX, FW, dK, partials);
+                               float* partials, size_t pstride, int nb, hipStream_t s) {
+    k_param_partials<<<dim3(diff_param_blocks(kW), nb), 256, 0, s>>>(Yc, kW, kX, FW, dK, partials,
+                                                                     pstride);
     return hipGetLastError();
 }
 
