@@ -205,6 +205,7 @@ template <int N1, int N2, bool INV>
 __device__ __forceinline__ void dft_pfa(float2* v) {
     constexpr int N = N1 * N2;
     constexpr int q1 = inv_mod(N2 % N1, N1), q2 = inv_mod(N1 % N2, N2);
+    static_assert(N2 * q1 % N1 == 1 && N1 * q2 % N2 == 1, "prime-factor lengths are coprime");
     float2 y[N];  // y[k1 * N2 + n2]
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) {
