@@ -68,8 +68,9 @@ struct DScale {
     float2* Q = nullptr;   // 8 x FH x FW bank spectra
     float* G = nullptr;    // 5 x FW x FH quadrature bank Re(Q_o) / (FH*FW), kx-major
     int Hc = 0, Wc = 0, ldc = 0;  // complex-cell region of the forward (as in detection)
-    float2* DT = nullptr;  // forward: (FW/2+1) x FH half spectrum, column-major
-    float* Cf = nullptr;   // forward: 8 x Hc x ldc complex-cell maps
+    float2* DT = nullptr;  // forward: (FW/2+1) x FH half spectrum, column-major, per image
+    float* Cf = nullptr;   // forward: 8 x Hc x ldc complex-cell maps, per image
+    int nbf = 1;           // forward: images per launch of its group (DT, Cf copies)
     float* dK = nullptr;   // 4 x 8 x kW x kW tap derivatives (dKe/dr, dKo/dr, dKe/dg, dKo/dg)
     size_t respOff = 0;    // offset of this scale's D map in a response vector
 };
@@ -202,32 +203,52 @@ int scale_forward(bimp_diff* d, DScale& x, int nb, hipStream_t s) {
 // row FFTs of the float level (two rows per complex FFT), column FFTs of the half spectra,
 // inverse columns with the quadrature bank G, inverse rows with the complex-cell magnitude
 // over the Hc x Wc region, then the cells stencil.
-int group_forward(bimp_diff* d, int first, int n, float* resp, hipStream_t s) {
+int group_forward(bimp_diff* d, int first, int n, int img0, int nb, float* resp, hipStream_t s) {
+    // entry k = b * n + j of the launches: image img0 + b of the pyramids in d->lev, scale
+    // first + j; its D map goes to resp + b * respSize
     const DScale& x0 = d->sc[first];
     const auto& g = x0.g;
+    const size_t dtSize = (size_t)(g.FW / 2 + 1) * g.FH;
     bimpk::ScaleSet ss;
-    ss.n = n;
+    ss.n = n * nb;
     bimpk::CellScale cs[bimpk::kCellMaxScales];
-    for (int k = 0; k < n; ++k) {
-        const DScale& x = d->sc[first + k];
-        ss.kX[k] = x.g.kX;
-        ss.G[k] = x.G;
-        ss.DT[k] = x.DT;
-        ss.C[k] = x.Cf;
-        cs[k] = bimpk::CellScale{x.Cf, x.Hc, x.ldc, g.FH, g.FW, g.H, g.W, x.cp,
-                                 resp + x.respOff, d->Sf + x.respOff, d->maxslot};
-    }
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < n; ++j) {
+            const DScale& x = d->sc[first + j];
+            const int k = b * n + j;
+            ss.kX[k] = x.g.kX;
+            ss.G[k] = x.G;
+            ss.DT[k] = x.DT + (size_t)b * dtSize;
+            ss.C[k] = x.Cf + (size_t)b * kNumOri * x.Hc * x.ldc;
+            cs[k] = bimpk::CellScale{ss.C[k], x.Hc, x.ldc, g.FH, g.FW, g.H, g.W, x.cp,
+                                     resp + (size_t)b * d->respSize + x.respOff,
+                                     d->Sf + (size_t)b * d->respSize + x.respOff, d->maxslot};
+        }
     const int ncols = g.FW / 2 + 1;
     const size_t rstride = (size_t)g.H * ncols;
     const size_t tstride = (size_t)kNumOri * x0.Hc * g.FW;
-    DCHK(d, bimpk::launch_fwd_rows_f32(g.FW, d->lev[g.level], (size_t)d->lw[g.level], g.H, g.W, ss,
-                                       255.0f * d->opt.input_scale, d->R, ncols, rstride,
-                                       d->tw[g.FW], s));
+    const size_t lsz = (size_t)d->lw[g.level] * d->lh[g.level];
+    for (int b = 0; b < nb; ++b) {
+        // the row pass reads one image: its n entries, written to their slots of R
+        bimpk::ScaleSet sb;
+        sb.n = n;
+        for (int j = 0; j < n; ++j) {
+            sb.kX[j] = ss.kX[b * n + j];
+            sb.G[j] = ss.G[b * n + j];
+            sb.DT[j] = ss.DT[b * n + j];
+            sb.C[j] = ss.C[b * n + j];
+        }
+        DCHK(d, bimpk::launch_fwd_rows_f32(g.FW, d->lev[g.level] + (img0 + b) * lsz,
+                                           (size_t)d->lw[g.level],
+                                           g.H, g.W, sb, 255.0f * d->opt.input_scale,
+                                           d->R + (size_t)b * n * rstride, ncols, rstride,
+                                           d->tw[g.FW], s));
+    }
     DCHK(d, bimpk::launch_fwd_cols(g.FH, d->R, ncols, rstride, g.H, ss, ncols, d->tw[g.FH], s));
     DCHK(d, bimpk::launch_inv_cols(g.FH, ss, g.FW, x0.Hc, d->T, tstride, d->tw[g.FH], s));
     DCHK(d, bimpk::launch_inv_rows(g.FW, g.FH, ss, d->T, tstride, x0.Hc, x0.Wc, x0.ldc,
                                    d->tw[g.FW], s));
-    DCHK(d, bimpk::launch_cells(cs, n, s));
+    DCHK(d, bimpk::launch_cells(cs, n * nb, s));
     return BIMP_OK;
 }
 
@@ -295,8 +316,11 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
                d->sc[i + n].g.FW == g.FW)
             ++n;
         d->groups.emplace_back(i, n);
-        d->rSize = std::max(d->rSize, (size_t)n * g.H * (g.FW / 2 + 1));
-        d->tSize = std::max(d->tSize, (size_t)n * kNumOri * d->sc[i].Hc * g.FW);
+        // the forward takes nbf images of the group per launch: n * nbf entries, at most 8
+        const int nbf = std::max(1, bimpk::kMaxBatch / n);
+        for (int j = 0; j < n; ++j) d->sc[i + j].nbf = nbf;
+        d->rSize = std::max(d->rSize, (size_t)n * nbf * g.H * (g.FW / 2 + 1));
+        d->tSize = std::max(d->tSize, (size_t)n * nbf * kNumOri * d->sc[i].Hc * g.FW);
         i += n;
     }
     d->lw.assign(d->maxLevel + 1, 0);
@@ -333,8 +357,8 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         d->maxBlocks = std::max(d->maxBlocks, bimpk::diff_param_blocks(x.g.kW));
         if (hipMalloc(&x.Q, (size_t)kNumOri * x.g.FH * x.g.FW * sizeof(float2)) != hipSuccess ||
             hipMalloc(&x.G, (size_t)bimpk::kUniqueOri * x.g.FH * x.g.FW * sizeof(float)) != hipSuccess ||
-            hipMalloc(&x.DT, (size_t)(x.g.FW / 2 + 1) * x.g.FH * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&x.Cf, (size_t)kNumOri * x.Hc * x.ldc * sizeof(float)) != hipSuccess ||
+            hipMalloc(&x.DT, (size_t)x.nbf * (x.g.FW / 2 + 1) * x.g.FH * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&x.Cf, (size_t)x.nbf * kNumOri * x.Hc * x.ldc * sizeof(float)) != hipSuccess ||
             hipMalloc(&x.dK, 4 * kNumOri * kk * sizeof(float)) != hipSuccess)
             return bail(BIMP_E_ALLOC);
     }
@@ -349,7 +373,7 @@ int bimp_diff_create(int width, int height, const float* lambdas, int n, const b
         hipMalloc(&d->maxslot, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&d->R, d->rSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->T, d->tSize * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&d->Sf, d->respSize * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d->Sf, (size_t)bimpk::kMaxBatch * d->respSize * sizeof(float)) != hipSuccess ||
         hipMalloc(&d->Yc, NB * d->ycSize * sizeof(float2)) != hipSuccess ||
         hipMalloc(&d->partials, NB * 2 * (size_t)d->maxBlocks * d->sc.size() * sizeof(float)) !=
                 hipSuccess)
@@ -408,12 +432,18 @@ int bimp_diff_forward(bimp_diff* d, const float* d_img, int nimg, float* d_resp,
     if (!d || !d_img || !d_resp || nimg <= 0) return BIMP_E_ARG;
     hipStream_t s = (hipStream_t)stream;
     const size_t isz = (size_t)d->width * d->height;
-    for (int b = 0; b < nimg; ++b) {
-        int st = pyramid(d, d_img + b * isz, 1, s);
+    // chunks of images whose pyramids fit d->lev; each group takes nbf of them per launch
+    for (int b0 = 0; b0 < nimg; b0 += d->nbMax) {
+        const int nb = std::min(d->nbMax, nimg - b0);
+        int st = pyramid(d, d_img + b0 * isz, nb, s);
         if (st) return st;
-        for (const auto& gr : d->groups)
-            if ((st = group_forward(d, gr.first, gr.second, d_resp + b * d->respSize, s)))
-                return st;
+        for (const auto& gr : d->groups) {
+            const int nbf = d->sc[gr.first].nbf;
+            for (int b = 0; b < nb; b += nbf)
+                if ((st = group_forward(d, gr.first, gr.second, b, std::min(nbf, nb - b),
+                                        d_resp + (size_t)(b0 + b) * d->respSize, s)))
+                    return st;
+        }
     }
     return BIMP_OK;
 }
