@@ -202,8 +202,14 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
                hipStream_t s, int part = kPartAll) {
     const int W0 = c->width, H0 = c->height;
     if (part == kPartAll || part == kPartHead) {
-        LaunchScope ls(c, s, F_INIT, 0);
-        HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
+        // the cells stage takes maxima into the per-scale slots, which must start at zero:
+        // the keypoint pass clears them after reading them, so only a frame after work that
+        // left them holding maxima needs the reset (a graph's head always takes it)
+        if (part == kPartHead || !c->slotsClean) {
+            LaunchScope ls(c, s, F_INIT, 0);
+            HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
+        }
+        c->slotsClean = true;
         c->maxValid = false;
         if (part == kPartHead) return BIMP_OK;
     }
@@ -382,14 +388,22 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
         HIPCHK(c, hipEventRecord(c->evJoin, c->side));
         HIPCHK(c, hipStreamWaitEvent(s, c->evJoin, 0));
     }
-    if (cells) c->maxValid = true;
+    if (cells) {
+        c->maxValid = true;
+        c->slotsClean = false;
+    }
     return BIMP_OK;
 }
 
 int run_cells(bimp_ctx* c, hipStream_t s) {
+    if (!c->slotsClean) {
+        LaunchScope ls(c, s, F_INIT, 0);
+        HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
+    }
     const int st = launch_cells_group(c, 0, (int)c->sc.size(), s);
     if (st) return st;
     c->maxValid = true;
+    c->slotsClean = false;
     return BIMP_OK;
 }
 
@@ -417,9 +431,12 @@ int run_keypoints(bimp_ctx* c, void* out, int cap, int matrix_ld, hipStream_t s)
     {
         LaunchScope ls(c, s, F_NMS_WRITE, 4.0 * bimpk::kNmsThreads * c->totalBlocks);
         HIPCHK(c, bimpk::launch_nms_write(c->nmsTable, (int)c->sc.size(), c->totalBlocks,
-                                          c->blockCounts, c->counter, c->blockMasks, out, cap,
-                                          matrix_ld, s));
+                                          c->blockCounts, c->counter, c->maxslots, c->blockMasks,
+                                          out, cap, matrix_ld, s));
     }
+    // the write pass cleared the slots: a later keypoint pass recomputes the maxima
+    c->maxValid = false;
+    c->slotsClean = true;
     return BIMP_OK;
 }
 
@@ -595,7 +612,9 @@ int run_frame_any(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, 
         const int st = build_frame_graph(c, src, p, out, cap, ld);
         if (st) return st;
     }
-    c->maxValid = true;
+    // the graph ends with the keypoint pass, which clears the max slots
+    c->maxValid = false;
+    c->slotsClean = true;
     HIPCHK(c, hipGraphLaunch(c->frameExec, s));
     return BIMP_OK;
 }
@@ -948,6 +967,9 @@ int bimp_create(int width, int height, const float* lambdas, int n, const bimp_o
         dalloc(&c->blockMasks, (size_t)blocks * bimpk::kNmsThreads) != hipSuccess ||
         dalloc(&c->maxslots, c->sc.size()) != hipSuccess || dalloc(&c->counter, 1) != hipSuccess)
         return bail(BIMP_E_ALLOC, "nms buffers");
+    if (hipMemset(c->maxslots, 0, c->sc.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(c->counter, 0, sizeof(int)) != hipSuccess)
+        return bail(BIMP_E_DEVICE, "nms buffers");
     if (hipMemcpy(c->nmsTable, tab.data(), tab.size() * sizeof(bimpk::NmsScale),
                   hipMemcpyHostToDevice) != hipSuccess)
         return bail(BIMP_E_DEVICE, "nms table upload");

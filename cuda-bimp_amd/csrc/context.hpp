@@ -93,6 +93,9 @@ struct bimp_ctx {
     uint8_t* hostIn = nullptr;
     hipEvent_t hostInDone = nullptr;
     bool maxValid = false;
+    // the per-scale max slots are zero at this point of the stream (set at creation and by
+    // the keypoint pass, which clears them once it has read them): a frame then needs no reset
+    bool slotsClean = true;
     std::string err;
     // profiling
     std::vector<bimpi::ProfEv> ev;

@@ -940,12 +940,13 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_count(const NmsScale* __res
 // Writes each keypoint of k_nms_count's hit masks at its row-major rank.  A block's first
 // rank is the sum of the counts of the blocks before it, summed here (a few thousand
 // cache-resident ints per block with keypoints) rather than by a scan launch; the last block
-// also stores the frame's total in counter.  matrix_ld > 0 selects the 7-row SURF_CUDA layout
-// (cuda_bimp.cuh:132-140) instead of keypoint structs.
+// also stores the frame's total in counter and clears the per-scale max slots (the count pass,
+// their only reader, is done), so that the next frame starts from zero.  matrix_ld > 0
+// selects the 7-row SURF_CUDA layout (cuda_bimp.cuh:132-140) instead of keypoint structs.
 __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
         const NmsScale* __restrict__ sc, int nscales, const int* __restrict__ counts,
-        int total_blocks, int* __restrict__ counter, const unsigned* __restrict__ masks,
-        void* __restrict__ out, int cap, int matrix_ld) {
+        int total_blocks, int* __restrict__ counter, unsigned* __restrict__ maxslots,
+        const unsigned* __restrict__ masks, void* __restrict__ out, int cap, int matrix_ld) {
 #pragma clang fp contract(off)
     const int blk = blockIdx.x;
     const bool last = blk == total_blocks - 1;
@@ -962,7 +963,10 @@ __global__ void __launch_bounds__(kNmsThreads) k_nms_write(
     int start = 0;
 #pragma unroll
     for (int i = 0; i < NW; ++i) start += ws[i];
-    if (last && threadIdx.x == 0) *counter = start + mine;
+    if (last) {
+        if (threadIdx.x == 0) *counter = start + mine;
+        for (int i = (int)threadIdx.x; i < nscales; i += kNmsThreads) maxslots[i] = 0u;
+    }
     // a block without keypoints, or whose first rank is past the capacity, writes nothing
     if (mine == 0 || start >= cap) return;  // uniform
     const int s = find_scale(sc, nscales, blk);
@@ -1434,11 +1438,13 @@ hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_block
 }
 
 hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_blocks,
-                            const int* block_counts, int* counter, const unsigned* masks,
-                            void* out, int cap, int matrix_ld, hipStream_t s) {
+                            const int* block_counts, int* counter, unsigned* maxslots,
+                            const unsigned* masks, void* out, int cap, int matrix_ld,
+                            hipStream_t s) {
     if (total_blocks < 1) return hipErrorInvalidValue;
     k_nms_write<<<total_blocks, kNmsThreads, 0, s>>>(scales, nscales, block_counts, total_blocks,
-                                                     counter, masks, out, cap, matrix_ld);
+                                                     counter, maxslots, masks, out, cap,
+                                                     matrix_ld);
     return hipGetLastError();
 }
 

@@ -139,13 +139,15 @@ hipError_t launch_cells(const CellScale* scales, int n, hipStream_t s);
 hipError_t launch_max(const float* D, int n, unsigned* maxslot, hipStream_t s);
 // k_nms_count stores each block's keypoint count and each thread's hit mask (total_blocks x
 // kNmsThreads words) for k_nms_write, which ranks the keypoints from the counts, stores the
-// total in counter and reads the response maps only around the hits.
+// total in counter, reads the response maps only around the hits and clears the max slots
+// (one per scale, nscales of them).
 hipError_t launch_nms_count(const NmsScale* scales, int nscales, int total_blocks,
                             const unsigned* maxslots, double thresh, int* block_counts,
                             unsigned* masks, hipStream_t s);
 hipError_t launch_nms_write(const NmsScale* scales, int nscales, int total_blocks,
-                            const int* block_counts, int* counter, const unsigned* masks,
-                            void* out, int cap, int matrix_ld, hipStream_t s);
+                            const int* block_counts, int* counter, unsigned* maxslots,
+                            const unsigned* masks, void* out, int cap, int matrix_ld,
+                            hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_bgr_to_gray(const uint8_t* bgr, size_t bpitch, int w, int h, uint8_t* gray,
                               size_t gpitch, hipStream_t s);

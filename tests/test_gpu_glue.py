@@ -117,10 +117,10 @@ def test_two_streams_equal_one_stream(B, w, h, form):
 @pytest.mark.parametrize("flags_name", ["two_streams", "single_stream"])
 def test_profile_counts_every_launch(B, flags_name):
     """With BIMP_FLAG_PROFILE every launch of a frame is timed and counted, whether it ran on
-    the caller's stream or the context's second one: once per frame for the clearing and the
-    three NMS passes, per pyramid level for the pyramid, and for the cells stage once per
-    inverse sub-batch (the scales whose maps become ready together share a launch), each with
-    a finite positive duration."""
+    the caller's stream or the context's second one: once per frame for the two NMS passes,
+    per pyramid level for the pyramid, and for the cells stage once per inverse sub-batch (the
+    scales whose maps become ready together share a launch), each with a finite positive
+    duration; no separate clearing of the max slots (each keypoint pass clears them)."""
     import math
 
     import oracle_lib as O
@@ -139,7 +139,9 @@ def test_profile_counts_every_launch(B, flags_name):
     prof = cs.profile()
     levels = max(si.level for si in cs.ks)
     count = {k: v["launches"] for k, v in prof.items()}
-    assert count["k_init"] == frames
+    # the max slots are cleared by each frame's keypoint pass: back-to-back detections need no
+    # separate reset
+    assert count.get("k_init", 0) == 0
     assert count["k_pyr_down"] == frames * levels
     inv_rows = sum(v for k, v in count.items() if k.startswith("k_inv_rows"))
     assert count["k_cells"] == inv_rows and frames <= inv_rows <= frames * len(cs.ks)
@@ -148,3 +150,31 @@ def test_profile_counts_every_launch(B, flags_name):
     assert any(k.startswith("k_inv_cols") for k in prof) and any(k.startswith("k_inv_rows") for k in prof)
     for k, v in prof.items():
         assert math.isfinite(v["total_ms"]) and v["total_ms"] > 0, k
+
+
+def test_max_slots_across_entry_points(B):
+    """The max slots are cleared by each keypoint pass and reset before cells only when left
+    holding maxima: detect, filter + cells + keypoints, keypoints again on the same maps, and
+    a graph frame in between all give the same keypoints as a fresh context."""
+    import oracle_lib as O
+    from kpmatch import bits
+    w, h = 320, 240
+    img = O.synth_frame(w, h, 25, 13)
+    ref, rn = B.CudaStuff(w, h, O.BASE_BANK).detect(img)
+    cs = B.CudaStuff(w, h, O.BASE_BANK)
+    for _ in range(2):
+        got, n = cs.detect(img)
+        assert n == rn and np.array_equal(bits(got), bits(ref))
+    cs.filter(img)
+    cs.cells()
+    for _ in range(2):  # the second pass recomputes the maxima the first one cleared
+        got, n = cs.keypoints()
+        assert n == rn and np.array_equal(bits(got), bits(ref))
+    cs.filter(img)
+    cs.cells()
+    cs.set_flags(B.FLAG_GRAPH)
+    got, n = cs.detect(img)
+    assert n == rn and np.array_equal(bits(got), bits(ref))
+    cs.set_flags(0)
+    got, n = cs.detect(img)
+    assert n == rn and np.array_equal(bits(got), bits(ref))
