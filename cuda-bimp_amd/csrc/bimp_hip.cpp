@@ -209,7 +209,9 @@ int run_filter(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, boo
             LaunchScope ls(c, s, F_INIT, 0);
             HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
         }
-        c->slotsClean = true;
+        // the cells stages below write the slots: dirty from their first launch on, so that a
+        // call failing part way leaves the next frame to reset them
+        c->slotsClean = !cells;
         c->maxValid = false;
         if (part == kPartHead) return BIMP_OK;
     }
@@ -400,10 +402,10 @@ int run_cells(bimp_ctx* c, hipStream_t s) {
         LaunchScope ls(c, s, F_INIT, 0);
         HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
     }
+    c->slotsClean = false;
     const int st = launch_cells_group(c, 0, (int)c->sc.size(), s);
     if (st) return st;
     c->maxValid = true;
-    c->slotsClean = false;
     return BIMP_OK;
 }
 
@@ -413,6 +415,7 @@ int run_keypoints(bimp_ctx* c, void* out, int cap, int matrix_ld, hipStream_t s)
             LaunchScope ls(c, s, F_INIT, 0);
             HIPCHK(c, bimpk::launch_init(c->maxslots, (int)c->sc.size(), c->counter, s));
         }
+        c->slotsClean = false;
         for (size_t i = 0; i < c->sc.size(); ++i) {
             auto& sc = c->sc[i];
             LaunchScope ls(c, s, F_MAX, 4.0 * sc.g.H * sc.g.W);
@@ -609,13 +612,17 @@ int run_frame_any(bimp_ctx* c, const uint8_t* img, size_t pitch, int on_device, 
     }
     if (!c->frameExec || c->gSrc != src || c->gPitch != p || c->gOut != out || c->gCap != cap ||
         c->gLd != ld || c->gFlags != c->opt.flags) {
+        // a capture runs nothing: the slots keep the state they had before it
+        const bool clean = c->slotsClean, valid = c->maxValid;
         const int st = build_frame_graph(c, src, p, out, cap, ld);
+        c->slotsClean = clean;
+        c->maxValid = valid;
         if (st) return st;
     }
+    HIPCHK(c, hipGraphLaunch(c->frameExec, s));
     // the graph ends with the keypoint pass, which clears the max slots
     c->maxValid = false;
     c->slotsClean = true;
-    HIPCHK(c, hipGraphLaunch(c->frameExec, s));
     return BIMP_OK;
 }
 
